@@ -1,0 +1,233 @@
+"""Benchmark: batched vs per-view sphere ray marching on one MI355X.
+
+Workload (default): 10M points from the seeded room generator (20 x 16 x 3 m, 800 boxes, 5 mm
+noise, 4x density variation, 0.5 % outliers, radii clamped at 0.05 m) and 256 views of mixed
+resolution (96 x 640x480, 96 x 1067x690, 64 x 1920x1080) = 232,879,680 rays per step.
+
+A step is one pass of the batched march over all 256 views (one persistent kernel launched as a
+HIP graph). The plain path (256 sequential per-view launches, the reference's launch shape) is
+timed beside it on the same inputs; the two must agree on every pixel. The kNN radii kernel is
+timed on the same cloud. Prints one JSON line (rank 0, one GPU).
+
+    python bench.py [--steps K] [--warmup W] [--workload room10m|room2m|box100k]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="room10m", choices=["room10m", "room2m", "box100k"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="use the default launch tune")
+    ap.add_argument("--plain-views", type=int, default=0,
+                    help="time the plain path on the first N views only (0 = all)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(views would be dealt to GPUs with no communication; see the design notes).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from renderer_ray_marching_amd import api, scenes
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    wl = {"room10m": scenes.workload_room_10m, "room2m": scenes.workload_room_2m,
+          "box100k": scenes.workload_box_100k}[a.workload]
+    t0 = time.time()
+    cloud, views, max_r = wl()
+    log(f"scene {a.workload}: {cloud.n} points, {len(views)} views ({time.time() - t0:.1f}s)")
+    rays = sum(v.width * v.height for v in views)
+
+    ctx = api.Context(0)
+    p = torch.from_numpy(cloud.xyzw()).cuda()
+    col = torch.from_numpy(cloud.colors4()).cuda()
+    t0 = time.time()
+    tree = api.KDTreeFlann(ctx)
+    tree.Build(p)
+    info = tree.info()
+    log(f"tree: {info} ({time.time() - t0:.1f}s)")
+
+    # ---- kNN radii kernel (k = 2, fused sqrt + clamp) ----
+    ctx.timing(True)
+    ru, rc = tree.radii(max_r)
+    ctx.sync()
+    radii_ms = []
+    for _ in range(5):
+        tree.radii(max_r)
+        radii_ms.append(ctx.last_kernel_ms())
+    radii_ms = float(np.median(radii_ms))
+    log(f"radii kernel: {radii_ms:.3f} ms")
+
+    marcher = api.PointcloudRayMarcher.get_instance(p, col, rc, tree)
+
+    # ---- plain per-view path: one launch per view, sequential ----
+    nplain = len(views) if a.plain_views <= 0 else min(a.plain_views, len(views))
+    plain_out = []
+    for v in views[:nplain]:
+        plain_out.append((torch.empty((v.height, v.width, 4), device="cuda"),
+                          torch.empty((v.height, v.width), dtype=torch.int32, device="cuda")))
+    poses = [v.pose_colmajor() for v in views]
+    marcher.render_to_texture(poses[0], views[0].fov, views[0].width, views[0].height,
+                              *plain_out[0])
+    ctx.sync()
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for i in range(nplain):
+        v = views[i]
+        marcher.render_to_texture(poses[i], v.fov, v.width, v.height, *plain_out[i])
+    ctx.sync()
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - tp
+    plain_rays = sum(v.width * v.height for v in views[:nplain])
+    plain_mrays = plain_rays / plain_s / 1e6
+    log(f"plain: {nplain} views, {plain_s:.3f}s, {plain_mrays:.1f} Mrays/s")
+
+    # ---- batched path: persistent kernel, tile queue, HIP graph ----
+    batch = api.Batch(marcher, views)
+    tune = batch.tune()
+    if not a.no_tune:
+        tt = time.time()
+        tune = batch.autotune(sample_stride=16)
+        log(f"autotune: {tune} ({time.time() - tt:.1f}s)")
+    for _ in range(a.warmup):
+        batch.launch()
+        ctx.last_kernel_ms()
+    ctx.sync()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        batch.launch()
+        kernel_ms.append(ctx.last_kernel_ms())
+        log(f"step: {kernel_ms[-1]:.1f} ms")
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall / a.steps * 1e3
+    value = rays * a.steps / wall / 1e6
+    avg_kernel_ms = float(np.mean(kernel_ms))
+
+    # every pixel of the batched output must equal the plain path
+    agree = 0
+    same_rgba = True
+    off = 0
+    for i in range(nplain):
+        v = views[i]
+        n = v.width * v.height
+        ph = plain_out[i][1].reshape(-1)
+        agree += int((batch.hit[off:off + n] == ph).sum().item())
+        same_rgba &= bool(torch.equal(batch.rgba[off:off + n], plain_out[i][0].reshape(-1, 4)))
+        off += n
+    hit_agreement = agree / plain_rays
+    hit_frac = float((batch.hit >= 0).float().mean().item())
+    log(f"batched: {value:.1f} Mrays/s, agreement with plain {hit_agreement:.6f}")
+
+    # ---- roofline (algorithmic bytes counted by the oracle, committed per workload) ----
+    alg_path = os.path.join(ROOT, "tests", "golden", "alg_bytes.json")
+    roofline = None
+    radii_roof = None
+    if os.path.exists(alg_path):
+        with open(alg_path) as f:
+            alg = json.load(f).get(a.workload)
+        if alg and alg["points"] == cloud.n and alg["rays"] == rays:
+            b_launch = alg["march_bytes_per_ray"] * rays
+            ach = b_launch / (avg_kernel_ms * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "batch_kernel (persistent march)",
+                        "alg_bytes_per_ray": round(alg["march_bytes_per_ray"], 1),
+                        "kernel_ms": round(avg_kernel_ms, 3)}
+            rb = alg["radii_bytes_total"]
+            rach = rb / (radii_ms * 1e-3) / 1e9
+            radii_roof = {"achieved": round(rach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(rach / HBM_PEAK_GBS, 4), "kernel_ms": round(radii_ms, 4),
+                          "alg_bytes_per_query": round(alg["radii_bytes_per_query"], 1),
+                          "compulsory_frac": round(16 * cloud.n / (radii_ms * 1e-3) / 1e9 /
+                                                   HBM_PEAK_GBS, 5)}
+
+    # ---- CPU baseline: the oracle's march on a bounded pixel sample, all host cores ----
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(cloud, views, rc.cpu().numpy())
+
+    out = {
+        "metric": "Mrays/s batched vs per-view, 10M pts, >=99.9% hit match; fwd+bwd ms/view",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded room generator)",
+        "config": {"workload": f"{a.workload}: {cloud.n} points, {len(views)} views, "
+                               f"{rays} rays per step", "points": cloud.n, "views": len(views),
+                   "rays_per_step": rays, "max_radius": max_r, "parallelism": "single GPU",
+                   "tree": info},
+        "plain_mrays_s": round(plain_mrays, 2),
+        "plain_views_timed": nplain,
+        "batched_vs_plain": round(value / plain_mrays, 3),
+        "hit_agreement_vs_plain": hit_agreement,
+        "rgba_identical_to_plain": same_rgba,
+        "hit_fraction": round(hit_frac, 4),
+        "tune": tune,
+        "radii_ms": round(radii_ms, 4),
+        "radii_roofline": radii_roof,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(cloud, views, radii):
+    """Oracle march (C++/OpenMP restatement of the reference algorithm) over every 256th pixel
+    of every view, on all host cores available to this process."""
+    from oracle import binding as ob
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(ncores, 64))
+    t0 = time.time()
+    tree = ob.RefTree(cloud.xyzw(), 64)
+    log(f"cpu baseline: oracle tree ({time.time() - t0:.1f}s)")
+    col = cloud.colors4()
+    stride = 256
+    rays = 0
+    tcpu = 0.0
+    for v in views:
+        pix = np.arange(0, v.width * v.height, stride, dtype=np.int32)
+        t1 = time.perf_counter()
+        tree.render(radii, col, v.pose_colmajor(), v.fov, v.width, v.height, pix=pix,
+                    nthreads=ncores)
+        tcpu += time.perf_counter() - t1
+        rays += pix.shape[0]
+    log(f"cpu baseline: {rays} rays in {tcpu:.1f}s")
+    return {"value": round(rays / tcpu / 1e6, 4), "unit": "Mrays/s", "cores": ncores,
+            "kind": "port",
+            "sample": f"every {stride}th pixel of all {len(views)} views ({rays} rays), "
+                      "oracle restatement of the reference march, OpenMP"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,171 @@
+/*
+ * librm_hip.so — MI355X (gfx950) kd-tree kNN, sphere radii and point-cloud sphere ray marching.
+ *
+ * C ABI. Every data pointer is a DEVICE pointer (the reference API takes
+ * thrust::device_vector for all of these), except rm_view arrays, poses and tuning
+ * structs, which are host memory. All work of a context is queued on the context's HIP stream;
+ * functions that return results to the host (rm_tree_build, the autotuner) synchronise it.
+ * Status: 0 on success, a negative RM_ERR_* code on failure; rm_last_error(ctx) then holds
+ * a message. No function calls exit() or throws across the ABI.
+ *
+ * Threading: one rm_ctx per host thread; calls on one ctx are serialised by its stream. There
+ * is no global device state: any number of trees, clouds and batches may coexist.
+ */
+#ifndef RM_H
+#define RM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RM_OK 0
+#define RM_ERR_INVALID (-1)     /* bad argument (null, size, range) */
+#define RM_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define RM_ERR_UNSUPPORTED (-3) /* e.g. k above RM_MAX_K */
+#define RM_ERR_NOMEM (-4)       /* device allocation failed */
+#define RM_ERR_INTERNAL (-5)    /* a device-side invariant failed (traversal stack overflow) */
+
+#define RM_MAX_K 32             /* largest k served by rm_knn */
+#define RM_MARCH_K 10           /* neighbours per march step (src/ray_marching.h:17) */
+
+typedef struct rm_ctx rm_ctx;
+typedef struct rm_tree rm_tree;
+typedef struct rm_cloud rm_cloud;
+typedef struct rm_batch rm_batch;
+
+/* ---- context -------------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with its own non-blocking stream. */
+int rm_ctx_create(int device, rm_ctx** out);
+int rm_ctx_destroy(rm_ctx* ctx);
+/* Queue work on a caller-owned stream instead (e.g. torch's current stream); NULL selects the
+ * device's null stream. Batch graphs are captured on the context's own stream either way. */
+int rm_ctx_set_stream(rm_ctx* ctx, void* hip_stream);
+int rm_ctx_synchronize(rm_ctx* ctx);
+const char* rm_last_error(const rm_ctx* ctx);
+/* Device time (ms) of the last kernel launched by rm_radii / rm_render / rm_batch_launch,
+ * measured with HIP events on the stream the kernel ran on (0 if events are off). */
+int rm_ctx_enable_timing(rm_ctx* ctx, int on);
+float rm_ctx_last_kernel_ms(const rm_ctx* ctx);
+
+/* ---- kd-tree --------------------------------------------------------------------------- */
+
+/* Build the search tree over n points given as n x 4 floats (x, y, z, w; w ignored, stride
+ * 16 B). Replaces kdtree::KDTreeFlann::Build (src/kdtree/kdtree_flann.cu:101-117) and
+ * KDTreeCuda3dIndex::buildIndex (third_party/flann/algorithms/kdtree_cuda_3d_index.h:124-141).
+ * leaf_max: points per leaf (the reference uses 64, kdtree_cuda_3d_index.h:55-61; 0 picks
+ * the library default). The tree copies the points; the input may be freed afterwards. */
+int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tree** out);
+int rm_tree_destroy(rm_tree* tree);
+int rm_tree_info(const rm_tree* tree, int64_t* n, int64_t* n_nodes, int* depth, int* leaf_max);
+
+/* Exact k nearest neighbours of m queries (m x 4 floats, w ignored). Outputs m x k, row-major,
+ * ascending by (squared distance, original index), padded with -1 / +INF when the tree holds
+ * fewer than k points. Replaces KDTreeFlann::Search / SearchKNN
+ * (src/kdtree/kdtree_flann.cu:5-28, 65-80) and KDTreeCuda3dIndex::knnSearchGpu
+ * (kdtree_cuda_3d_index.cu:261-399) including the index remap map_indices (:160-173).
+ * Squared distance is ((dx*dx + dy*dy) + dz*dz) in float, no fused multiply-add. */
+int rm_knn(rm_ctx* ctx, const rm_tree* tree, const float* q_xyzw, int64_t m, int k,
+           int32_t* idx, float* d2);
+
+/* Radius search: up to max_nn neighbours with d2 < radius^2 (strict, as the reference result
+ * set KnnRadiusResultSet, util/cuda/result_set.h:372-476) per query, ascending by
+ * (d2, index), padded with -1 / +INF; counts[i] = number found (<= max_nn, may be NULL).
+ * Replaces KDTreeFlann::SearchRadius (src/kdtree/kdtree_flann.cu:82-99) and radiusSearchGpu
+ * (kdtree_cuda_3d_index.cu:410-557). max_nn <= RM_MAX_K. */
+int rm_radius_search(rm_ctx* ctx, const rm_tree* tree, const float* q_xyzw, int64_t m,
+                     float radius, int max_nn, int32_t* idx, float* d2, int32_t* counts);
+
+/* Per-point sphere radius: r_i = sqrtf(second smallest squared distance from point i to the
+ * cloud, self included), i.e. the distance to the nearest other point (0 for duplicates).
+ * r_clamped_i = (max_radius > 0 && r_i > max_radius) ? max_radius : r_i. Both outputs are in
+ * the original point order; either may be NULL. Replaces main.cu:99-149 (k = 2 search, the
+ * host sqrt loop :110-112, and the clamp :144-149) as one fused kernel. */
+int rm_radii(rm_ctx* ctx, const rm_tree* tree, float max_radius, float* r_unclamped,
+             float* r_clamped);
+
+/* ---- cloud + march --------------------------------------------------------------------- */
+
+/* Bind colours and radii to a tree for marching. xyzw: n x 4 (the same points the tree was
+ * built from, original order), colors: n x 4 floats (r, g, b in [0,1], a ignored),
+ * radii: n floats (already clamped). The cloud copies what it needs.
+ * Replaces PointcloudRayMarcher::get_instance / its constructor
+ * (src/ray_marching.cu:159-200); unlike the reference it is not a singleton. */
+int rm_cloud_create(rm_ctx* ctx, const rm_tree* tree, const float* xyzw, const float* colors,
+                    const float* radii, int64_t n, rm_cloud** out);
+int rm_cloud_destroy(rm_cloud* cloud);
+/* Replace the radii (n floats, original order) of a cloud, e.g. after optimisation. */
+int rm_cloud_set_radii(rm_ctx* ctx, rm_cloud* cloud, const float* radii);
+/* Replace the colours (n x 4 floats) of a cloud. */
+int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* cloud, const float* colors);
+/* Move the points of a cloud (xyz: n x 3 floats, original order) and refit the boxes of the
+ * tree it was created from, keeping the topology. Search stays exact because every bound is
+ * recomputed from the moved points. The tree is modified in place. */
+int rm_cloud_refit(rm_ctx* ctx, rm_cloud* cloud, const float* xyz);
+
+/* Render one view. pose: 16 floats, column-major camera-to-world (OpenGL, -z forward), as the
+ * glm::mat4 of render_to_texture. fov_radians: horizontal field of view. Output rgba: H x W x 4
+ * floats, row 0 = bottom (GL texture order); a hit writes (r, g, b, depth = ray length), a
+ * miss (0, 0, 0, 0). hit: H x W int32 original point index, -1 on a miss (may be NULL).
+ * Replaces PointcloudRayMarcher::render_to_texture (src/ray_marching.cu:93-154) and the
+ * render kernel (:63-91, 202-228, 260-292). */
+int rm_render(rm_ctx* ctx, const rm_cloud* cloud, const float* pose, float fov_radians, int W,
+              int H, float* rgba, int32_t* hit);
+
+/* ---- batched march (no counterpart in the reference) ------------------------------------ */
+
+typedef struct rm_view {
+    float pose[16];    /* column-major camera-to-world */
+    float fov_radians;
+    int32_t width, height;
+    int32_t reserved;
+    int64_t offset;    /* first pixel of this view in the concatenated outputs */
+} rm_view;
+
+/* Launch parameters picked by the autotuner. */
+typedef struct rm_tune {
+    int32_t tile_w, tile_h;   /* pixel tile per wavefront packet, tile_w * tile_h == 64 */
+    int32_t mode;             /* 0 = packet traversal (wave-uniform), 1 = per-lane traversal */
+    int32_t warm;             /* 1 = warm-start bound from the previous step's neighbours */
+    int32_t waves_per_block;  /* 1, 2 or 4 */
+    int32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu */
+} rm_tune;
+
+/* A batch of views marched by one persistent kernel over a shared tile queue, launched as a
+ * captured HIP graph. rgba / hit (hit may be NULL) are the concatenated outputs, sized by the
+ * views' offsets and sizes; they are bound at creation (the graph captures them). */
+int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
+                    const rm_tune* tune, float* rgba, int32_t* hit, rm_batch** out);
+/* Time candidate rm_tune settings on a 1/sample_stride subset of the tiles and keep the
+ * fastest (re-captures the graph). chosen may be NULL. */
+int rm_batch_autotune(rm_batch* batch, int sample_stride, rm_tune* chosen);
+int rm_batch_get_tune(const rm_batch* batch, rm_tune* tune);
+int rm_batch_launch(rm_batch* batch);
+int rm_batch_destroy(rm_batch* batch);
+/* One-call form: create with the default tune, launch once, destroy. */
+int rm_render_batch(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
+                    float* rgba, int32_t* hit);
+
+/* ---- differentiable march (no counterpart in the reference) ----------------------------- */
+
+/* Forward of the differentiable form for one view: the plain march picks the hit point h of
+ * each pixel (hit, -1 on a miss); then rgb = colour of h (H x W x 3) and depth = analytic
+ * entry distance of the ray into the sphere (centre, radius) of h (H x W), or the marched
+ * distance where the ray only grazes that sphere (discriminant <= 0). Misses give 0. */
+int rm_render_diff(rm_ctx* ctx, const rm_cloud* cloud, const float* pose, float fov_radians,
+                   int W, int H, float* rgb, float* depth, int32_t* hit);
+
+/* Adjoint of rm_render_diff with the hit map held fixed. grad_rgb: H x W x 3, grad_depth:
+ * H x W upstream gradients. Accumulates (adds) into g_xyz (n x 3), g_r (n), g_rgb (n x 3) in
+ * the original point order. Per-point sums are formed by sorting the hit pixels by point
+ * index and reducing each run, not by atomics on the point arrays. */
+int rm_render_adjoint(rm_ctx* ctx, const rm_cloud* cloud, const float* pose, float fov_radians,
+                      int W, int H, const int32_t* hit, const float* grad_rgb,
+                      const float* grad_depth, float* g_xyz, float* g_r, float* g_rgb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RM_H */

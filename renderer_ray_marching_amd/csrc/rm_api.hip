@@ -1,0 +1,760 @@
+// C ABI of librm_hip.so (declared in include/rm.h): contexts, trees, clouds, the plain and the
+// batched march (persistent kernel, HIP graph, autotuner) and the differentiable form.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rm.h"
+#include "rm_internal.h"
+
+using rm::DevCloud;
+using rm::DevTree;
+using rm::DevView;
+
+struct rm_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int* d_err = nullptr;          // device-side invariant flag
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ev_pending = false;
+    int num_cus = 0;
+    void* ws = nullptr;            // grow-only workspace
+    size_t ws_bytes = 0;
+};
+
+struct rm_tree {
+    int device = 0;
+    int64_t n = 0;
+    int64_t n_nodes = 0;
+    int depth = 0;
+    int leaf_max = 0;
+    RmNode* nodes = nullptr;
+    float4* pts = nullptr;
+    int32_t* level_order = nullptr;        // inner node ids grouped by level (device)
+    std::vector<int32_t> level_start;      // host: offsets into level_order, nlevels + 1
+};
+
+struct rm_cloud {
+    rm_tree* tree = nullptr;
+    int64_t n = 0;
+    float4* xyzr = nullptr;
+    float4* colors = nullptr;
+};
+
+struct rm_batch {
+    rm_ctx* ctx = nullptr;
+    const rm_cloud* cloud = nullptr;
+    std::vector<rm_view> views;
+    DevView* d_views = nullptr;
+    int64_t total_tiles = 0;
+    int64_t total_pixels = 0;
+    rm_tune tune{};
+    unsigned int* d_queue = nullptr;
+    float* rgba = nullptr;
+    int32_t* hit = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int blocks = 0;
+};
+
+namespace {
+
+const int kDefaultLeaf = 16;
+
+int fail(rm_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(rm_ctx* ctx, hipError_t e, const char* where) {
+    return fail(ctx, RM_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define RM_HIP(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+DevTree dev_tree(const rm_tree* t) {
+    DevTree d;
+    d.nodes = t->nodes;
+    d.pts = t->pts;
+    d.n = t->n;
+    d.depth = t->depth;
+    return d;
+}
+
+DevCloud dev_cloud(const rm_cloud* c) {
+    DevCloud d;
+    d.tree = dev_tree(c->tree);
+    d.xyzr = c->xyzr;
+    d.colors = c->colors;
+    return d;
+}
+
+// Pane distance of src/ray_marching.cu:74, computed once on the host (float, libm tanf).
+float pane_distance(int W, float fov) { return (float)W / (2.0f * std::tan(0.5f * fov)); }
+
+DevView dev_view(const float* pose, float fov, int W, int H, int64_t pix_offset, int tile_w,
+                 int64_t tile_offset) {
+    DevView v;
+    std::memcpy(v.M, pose, sizeof(v.M));
+    v.pane = pane_distance(W, fov);
+    v.W = W;
+    v.H = H;
+    v.tiles_x = tile_w > 0 ? (W + tile_w - 1) / tile_w : 0;
+    v.pix_offset = pix_offset;
+    v.tile_offset = tile_offset;
+    return v;
+}
+
+int set_device(rm_ctx* ctx) {
+    RM_HIP(ctx, hipSetDevice(ctx->device));
+    return RM_OK;
+}
+
+int ensure_ws(rm_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->ws_bytes) return RM_OK;
+    if (ctx->ws) {
+        RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        RM_HIP(ctx, hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    size_t b = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc(&ctx->ws, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, RM_ERR_NOMEM, "workspace allocation failed");
+    }
+    ctx->ws_bytes = b;
+    return RM_OK;
+}
+
+void timing_begin(rm_ctx* ctx) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
+}
+void timing_end(rm_ctx* ctx) {
+    if (ctx->timing) {
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        ctx->ev_pending = true;
+    }
+}
+
+int check_tree(rm_ctx* ctx, const rm_tree* t) {
+    if (!t) return fail(ctx, RM_ERR_INVALID, "null tree");
+    if (t->device != ctx->device) return fail(ctx, RM_ERR_INVALID, "tree is on another device");
+    return RM_OK;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------------------
+int rm_ctx_create(int device, rm_ctx** out) {
+    if (!out) return RM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        (void)hipGetLastError();
+        return RM_ERR_INVALID;
+    }
+    rm_ctx* ctx = new (std::nothrow) rm_ctx();
+    if (!ctx) return RM_ERR_NOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->d_err, sizeof(int)) != hipSuccess ||
+        hipMemset(ctx->d_err, 0, sizeof(int)) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        rm_ctx_destroy(ctx);
+        return RM_ERR_HIP;
+    }
+    ctx->stream = ctx->own;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (ctx->num_cus <= 0) ctx->num_cus = 256;
+    *out = ctx;
+    return RM_OK;
+}
+
+int rm_ctx_destroy(rm_ctx* ctx) {
+    if (!ctx) return RM_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return RM_OK;
+}
+
+int rm_ctx_set_stream(rm_ctx* ctx, void* hip_stream) {
+    if (!ctx) return RM_ERR_INVALID;
+    ctx->stream = (hipStream_t)hip_stream;
+    return RM_OK;
+}
+
+int rm_ctx_synchronize(rm_ctx* ctx) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (int r = set_device(ctx)) return r;
+    RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int flag = 0;
+    RM_HIP(ctx, hipMemcpy(&flag, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        (void)hipMemset(ctx->d_err, 0, sizeof(int));
+        return fail(ctx, RM_ERR_INTERNAL, "device traversal stack overflow");
+    }
+    return RM_OK;
+}
+
+const char* rm_last_error(const rm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rm_ctx_enable_timing(rm_ctx* ctx, int on) {
+    if (!ctx) return RM_ERR_INVALID;
+    ctx->timing = on != 0;
+    ctx->ev_pending = false;
+    return RM_OK;
+}
+
+float rm_ctx_last_kernel_ms(const rm_ctx* ctx) {
+    if (!ctx || !ctx->timing || !ctx->ev_pending) return 0.0f;
+    if (hipEventSynchronize(ctx->ev1) != hipSuccess) return 0.0f;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) return 0.0f;
+    return ms;
+}
+
+// ---------------------------------------------------------------------------------------------
+// tree
+// ---------------------------------------------------------------------------------------------
+int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tree** out) {
+    if (!ctx || !out) return RM_ERR_INVALID;
+    *out = nullptr;
+    if (!xyzw || n <= 0) return fail(ctx, RM_ERR_INVALID, "rm_tree_build: empty input");
+    if (leaf_max == 0) leaf_max = kDefaultLeaf;
+    if (int r = set_device(ctx)) return r;
+    std::vector<float> host((size_t)n * 4);
+    RM_HIP(ctx, hipMemcpyAsync(host.data(), xyzw, (size_t)n * 16, hipMemcpyDefault, ctx->stream));
+    RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RmHostTree ht;
+    std::string msg;
+    if (!rm_build_host_tree(host.data(), n, leaf_max, ht, msg)) return fail(ctx, RM_ERR_INVALID, msg);
+    if (ht.depth > rm::kMaxDepth)
+        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: tree deeper than supported");
+    rm_tree* t = new (std::nothrow) rm_tree();
+    if (!t) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");
+    t->device = ctx->device;
+    t->n = n;
+    t->n_nodes = (int64_t)ht.nodes.size();
+    t->depth = ht.depth;
+    t->leaf_max = leaf_max;
+    // level grouping for refits
+    int nlev = 0;
+    for (int32_t l : ht.level) nlev = std::max(nlev, l + 1);
+    t->level_start.assign(nlev + 1, 0);
+    for (int32_t l : ht.level) t->level_start[l + 1]++;
+    for (int l = 0; l < nlev; ++l) t->level_start[l + 1] += t->level_start[l];
+    std::vector<int32_t> order(ht.level.size());
+    {
+        std::vector<int32_t> fillp(t->level_start.begin(), t->level_start.end() - 1);
+        for (size_t i = 0; i < ht.level.size(); ++i) order[fillp[ht.level[i]]++] = (int32_t)i;
+    }
+    if (hipMalloc(&t->nodes, sizeof(RmNode) * ht.nodes.size()) != hipSuccess ||
+        hipMalloc(&t->pts, sizeof(float4) * (size_t)n) != hipSuccess ||
+        hipMalloc(&t->level_order, sizeof(int32_t) * std::max<size_t>(1, order.size())) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        rm_tree_destroy(t);
+        return fail(ctx, RM_ERR_NOMEM, "rm_tree_build: device allocation failed");
+    }
+    hipError_t e = hipMemcpyAsync(t->nodes, ht.nodes.data(), sizeof(RmNode) * ht.nodes.size(),
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->pts, ht.pts.data(), sizeof(float4) * (size_t)n,
+                           hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && !order.empty())
+        e = hipMemcpyAsync(t->level_order, order.data(), sizeof(int32_t) * order.size(),
+                           hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        rm_tree_destroy(t);
+        return hip_fail(ctx, e, "rm_tree_build upload");
+    }
+    *out = t;
+    return RM_OK;
+}
+
+int rm_tree_destroy(rm_tree* t) {
+    if (!t) return RM_OK;
+    (void)hipSetDevice(t->device);
+    if (t->nodes) (void)hipFree(t->nodes);
+    if (t->pts) (void)hipFree(t->pts);
+    if (t->level_order) (void)hipFree(t->level_order);
+    delete t;
+    return RM_OK;
+}
+
+int rm_tree_info(const rm_tree* t, int64_t* n, int64_t* n_nodes, int* depth, int* leaf_max) {
+    if (!t) return RM_ERR_INVALID;
+    if (n) *n = t->n;
+    if (n_nodes) *n_nodes = t->n_nodes;
+    if (depth) *depth = t->depth;
+    if (leaf_max) *leaf_max = t->leaf_max;
+    return RM_OK;
+}
+
+int rm_knn(rm_ctx* ctx, const rm_tree* tree, const float* q, int64_t m, int k, int32_t* idx,
+           float* d2) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (int r = check_tree(ctx, tree)) return r;
+    if (k < 1) return fail(ctx, RM_ERR_INVALID, "rm_knn: k must be >= 1");
+    if (k > RM_MAX_K) return fail(ctx, RM_ERR_UNSUPPORTED, "rm_knn: k above RM_MAX_K");
+    if (m < 0 || (m > 0 && (!q || !idx || !d2)))
+        return fail(ctx, RM_ERR_INVALID, "rm_knn: bad buffers");
+    if (int r = set_device(ctx)) return r;
+    RM_HIP(ctx, rm::launch_knn(ctx->stream, dev_tree(tree), q, m, k, idx, d2, ctx->d_err));
+    return RM_OK;
+}
+
+int rm_radius_search(rm_ctx* ctx, const rm_tree* tree, const float* q, int64_t m, float radius,
+                     int max_nn, int32_t* idx, float* d2, int32_t* counts) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (int r = check_tree(ctx, tree)) return r;
+    if (max_nn < 1) return fail(ctx, RM_ERR_INVALID, "rm_radius_search: max_nn must be >= 1");
+    if (max_nn > RM_MAX_K)
+        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_radius_search: max_nn above RM_MAX_K");
+    if (!(radius >= 0.0f)) return fail(ctx, RM_ERR_INVALID, "rm_radius_search: bad radius");
+    if (m < 0 || (m > 0 && (!q || !idx || !d2)))
+        return fail(ctx, RM_ERR_INVALID, "rm_radius_search: bad buffers");
+    if (int r = set_device(ctx)) return r;
+    float r2 = radius * radius;  // float(radius * radius), src/kdtree/kdtree_flann.cu:98
+    RM_HIP(ctx, rm::launch_radius(ctx->stream, dev_tree(tree), q, m, r2, max_nn, idx, d2, counts,
+                                  ctx->d_err));
+    return RM_OK;
+}
+
+int rm_radii(rm_ctx* ctx, const rm_tree* tree, float max_radius, float* r_unc, float* r_cl) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (int r = check_tree(ctx, tree)) return r;
+    if (int r = set_device(ctx)) return r;
+    if (!r_unc && !r_cl) return RM_OK;
+    timing_begin(ctx);
+    RM_HIP(ctx, rm::launch_radii(ctx->stream, dev_tree(tree), max_radius, r_unc, r_cl, ctx->d_err));
+    timing_end(ctx);
+    return RM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// cloud
+// ---------------------------------------------------------------------------------------------
+int rm_cloud_create(rm_ctx* ctx, const rm_tree* tree, const float* xyzw, const float* colors,
+                    const float* radii, int64_t n, rm_cloud** out) {
+    if (!ctx || !out) return RM_ERR_INVALID;
+    *out = nullptr;
+    if (int r = check_tree(ctx, tree)) return r;
+    if (n != tree->n) return fail(ctx, RM_ERR_INVALID, "rm_cloud_create: n differs from the tree");
+    if (!xyzw || !colors || !radii) return fail(ctx, RM_ERR_INVALID, "rm_cloud_create: null input");
+    if (int r = set_device(ctx)) return r;
+    rm_cloud* c = new (std::nothrow) rm_cloud();
+    if (!c) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");
+    c->tree = const_cast<rm_tree*>(tree);
+    c->n = n;
+    if (hipMalloc(&c->xyzr, sizeof(float4) * (size_t)n) != hipSuccess ||
+        hipMalloc(&c->colors, sizeof(float4) * (size_t)n) != hipSuccess) {
+        (void)hipGetLastError();
+        rm_cloud_destroy(c);
+        return fail(ctx, RM_ERR_NOMEM, "rm_cloud_create: device allocation failed");
+    }
+    hipError_t e = rm::launch_make_cloud(ctx->stream, xyzw, radii, colors, n, c->xyzr, c->colors);
+    if (e != hipSuccess) {
+        rm_cloud_destroy(c);
+        return hip_fail(ctx, e, "rm_cloud_create");
+    }
+    *out = c;
+    return RM_OK;
+}
+
+int rm_cloud_destroy(rm_cloud* c) {
+    if (!c) return RM_OK;
+    if (c->tree) (void)hipSetDevice(c->tree->device);
+    if (c->xyzr) (void)hipFree(c->xyzr);
+    if (c->colors) (void)hipFree(c->colors);
+    delete c;
+    return RM_OK;
+}
+
+int rm_cloud_set_radii(rm_ctx* ctx, rm_cloud* c, const float* radii) {
+    if (!ctx || !c || !radii) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_radii: null input");
+    if (int r = set_device(ctx)) return r;
+    RM_HIP(ctx, rm::launch_set_radii(ctx->stream, radii, c->n, c->xyzr));
+    return RM_OK;
+}
+
+int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* c, const float* colors) {
+    if (!ctx || !c || !colors) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_colors: null input");
+    if (int r = set_device(ctx)) return r;
+    RM_HIP(ctx, rm::launch_set_colors(ctx->stream, colors, c->n, c->colors));
+    return RM_OK;
+}
+
+int rm_cloud_refit(rm_ctx* ctx, rm_cloud* c, const float* xyz) {
+    if (!ctx || !c || !xyz) return fail(ctx, RM_ERR_INVALID, "rm_cloud_refit: null input");
+    if (int r = set_device(ctx)) return r;
+    rm_tree* t = c->tree;
+    RM_HIP(ctx, rm::launch_refit(ctx->stream, t->nodes, t->pts, c->xyzr, xyz, t->n,
+                                 t->level_order, t->level_start.data(),
+                                 (int)t->level_start.size() - 1));
+    return RM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// plain march
+// ---------------------------------------------------------------------------------------------
+int rm_render(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov, int W, int H,
+              float* rgba, int32_t* hit) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (!c || !pose || !rgba) return fail(ctx, RM_ERR_INVALID, "rm_render: null input");
+    if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render: empty image");
+    if (!(fov > 0.0f && fov < 3.14159f)) return fail(ctx, RM_ERR_INVALID, "rm_render: bad fov");
+    if (int r = set_device(ctx)) return r;
+    DevView v = dev_view(pose, fov, W, H, 0, 0, 0);
+    timing_begin(ctx);
+    RM_HIP(ctx, rm::launch_render_plain(ctx->stream, dev_cloud(c), v,
+                                        rm::MarchOut{reinterpret_cast<float4*>(rgba), hit},
+                                        ctx->d_err));
+    timing_end(ctx);
+    return RM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched march
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+rm_tune default_tune() {
+    rm_tune t;
+    t.tile_w = 8;
+    t.tile_h = 8;
+    t.mode = 0;
+    t.warm = 1;
+    t.waves_per_block = 2;
+    t.blocks_per_cu = 0;
+    return t;
+}
+
+bool tune_valid(const rm_tune& t) {
+    return t.tile_w > 0 && t.tile_h > 0 && t.tile_w * t.tile_h == 64 && (t.mode == 0 || t.mode == 1) &&
+           (t.warm == 0 || t.warm == 1) &&
+           (t.waves_per_block == 1 || t.waves_per_block == 2 || t.waves_per_block == 4) &&
+           t.blocks_per_cu >= 0 && t.blocks_per_cu <= 64;
+}
+
+// Lay out the views' tiles for a tune and upload the view table.
+int batch_layout(rm_batch* b) {
+    rm_ctx* ctx = b->ctx;
+    std::vector<DevView> dv(b->views.size());
+    int64_t tiles = 0;
+    for (size_t i = 0; i < b->views.size(); ++i) {
+        const rm_view& v = b->views[i];
+        dv[i] = dev_view(v.pose, v.fov_radians, v.width, v.height, v.offset, b->tune.tile_w, tiles);
+        int64_t ty = (v.height + b->tune.tile_h - 1) / b->tune.tile_h;
+        tiles += (int64_t)dv[i].tiles_x * ty;
+    }
+    b->total_tiles = tiles;
+    if (tiles >= (int64_t)UINT32_MAX) return fail(ctx, RM_ERR_UNSUPPORTED, "too many tiles");
+    RM_HIP(ctx, hipMemcpyAsync(b->d_views, dv.data(), sizeof(DevView) * dv.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int bpc = b->tune.blocks_per_cu;
+    if (bpc == 0) {
+        // occupancy-derived persistent grid: as many resident blocks as fit
+        bpc = std::max(1, 16 / b->tune.waves_per_block);
+    }
+    b->blocks = ctx->num_cus * bpc;
+    return RM_OK;
+}
+
+rm::BatchParams batch_params(rm_batch* b, int64_t sample_stride) {
+    rm::BatchParams p;
+    p.cloud = dev_cloud(b->cloud);
+    p.views = b->d_views;
+    p.nviews = (int32_t)b->views.size();
+    p.tile_w = b->tune.tile_w;
+    p.tile_h = b->tune.tile_h;
+    p.total_tiles = b->total_tiles;
+    p.sample_stride = sample_stride;
+    p.queue = b->d_queue;
+    p.err = b->ctx->d_err;
+    p.out.rgba = reinterpret_cast<float4*>(b->rgba);
+    p.out.hit = b->hit;
+    return p;
+}
+
+int batch_enqueue(rm_batch* b, hipStream_t s, int64_t sample_stride) {
+    rm_ctx* ctx = b->ctx;
+    RM_HIP(ctx, rm::launch_reset_queue(s, b->d_queue));
+    RM_HIP(ctx, rm::launch_batch(s, batch_params(b, sample_stride), b->tune.mode, b->tune.warm,
+                                 b->tune.waves_per_block, b->blocks));
+    return RM_OK;
+}
+
+void batch_drop_graph(rm_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int batch_capture(rm_batch* b) {
+    rm_ctx* ctx = b->ctx;
+    batch_drop_graph(b);
+    // Capture on the context's own stream (the null stream cannot be captured); the graph is
+    // launched on whatever stream the context works on.
+    RM_HIP(ctx, hipStreamBeginCapture(ctx->own, hipStreamCaptureModeThreadLocal));
+    int r = batch_enqueue(b, ctx->own, 1);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->own, &g);
+    if (r != RM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    b->graph = g;
+    RM_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return RM_OK;
+}
+
+}  // namespace
+
+int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
+                    const rm_tune* tune, float* rgba, int32_t* hit, rm_batch** out) {
+    if (!ctx || !out) return RM_ERR_INVALID;
+    *out = nullptr;
+    if (!cloud || !views || nviews <= 0 || !rgba)
+        return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad arguments");
+    for (int i = 0; i < nviews; ++i) {
+        const rm_view& v = views[i];
+        if (v.width <= 0 || v.height <= 0 || v.offset < 0 ||
+            !(v.fov_radians > 0.0f && v.fov_radians < 3.14159f))
+            return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad view " + std::to_string(i));
+    }
+    rm_tune t = tune ? *tune : default_tune();
+    if (!tune_valid(t)) return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad tune");
+    if (t.warm && cloud->n < rm::kMarchK) t.warm = 0;
+    if (int r = set_device(ctx)) return r;
+    rm_batch* b = new (std::nothrow) rm_batch();
+    if (!b) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");
+    b->ctx = ctx;
+    b->cloud = cloud;
+    b->views.assign(views, views + nviews);
+    b->tune = t;
+    b->rgba = rgba;
+    b->hit = hit;
+    if (hipMalloc(&b->d_views, sizeof(DevView) * nviews) != hipSuccess ||
+        hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess) {
+        (void)hipGetLastError();
+        rm_batch_destroy(b);
+        return fail(ctx, RM_ERR_NOMEM, "rm_batch_create: device allocation failed");
+    }
+    int r = batch_layout(b);
+    if (r == RM_OK) r = batch_capture(b);
+    if (r != RM_OK) {
+        rm_batch_destroy(b);
+        return r;
+    }
+    *out = b;
+    return RM_OK;
+}
+
+int rm_batch_get_tune(const rm_batch* b, rm_tune* tune) {
+    if (!b || !tune) return RM_ERR_INVALID;
+    *tune = b->tune;
+    return RM_OK;
+}
+
+int rm_batch_launch(rm_batch* b) {
+    if (!b) return RM_ERR_INVALID;
+    rm_ctx* ctx = b->ctx;
+    if (int r = set_device(ctx)) return r;
+    timing_begin(ctx);
+    RM_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    timing_end(ctx);
+    return RM_OK;
+}
+
+int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
+    if (!b) return RM_ERR_INVALID;
+    rm_ctx* ctx = b->ctx;
+    if (sample_stride < 1) sample_stride = 1;
+    if (int r = set_device(ctx)) return r;
+    hipEvent_t e0, e1;
+    RM_HIP(ctx, hipEventCreate(&e0));
+    RM_HIP(ctx, hipEventCreate(&e1));
+    auto measure = [&](const rm_tune& t, float& ms) -> int {
+        b->tune = t;
+        if (int r = batch_layout(b)) return r;
+        float best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            RM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            if (int r = batch_enqueue(b, ctx->stream, sample_stride)) return r;
+            RM_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            RM_HIP(ctx, hipEventSynchronize(e1));
+            float v = 0.0f;
+            RM_HIP(ctx, hipEventElapsedTime(&v, e0, e1));
+            best = std::min(best, v);
+        }
+        ms = best;
+        return RM_OK;
+    };
+    rm_tune best = b->tune;
+    float best_ms = 1e30f;
+    int r = RM_OK;
+    auto consider = [&](rm_tune t) {
+        if (r != RM_OK || !tune_valid(t)) return;
+        if (t.warm && b->cloud->n < rm::kMarchK) return;
+        float ms;
+        r = measure(t, ms);
+        if (r == RM_OK && ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    };
+    // coordinate descent: traversal mode x warm start, then tile shape, then block shape
+    rm_tune base = default_tune();
+    for (int mode = 0; mode < 2; ++mode)
+        for (int warm = 0; warm < 2; ++warm) {
+            rm_tune t = base;
+            t.mode = mode;
+            t.warm = warm;
+            consider(t);
+        }
+    const int shapes[][2] = {{16, 4}, {4, 16}, {32, 2}};
+    rm_tune cur = best;
+    for (auto& sh : shapes) {
+        rm_tune t = cur;
+        t.tile_w = sh[0];
+        t.tile_h = sh[1];
+        consider(t);
+    }
+    cur = best;
+    for (int wpb : {1, 4}) {
+        rm_tune t = cur;
+        t.waves_per_block = wpb;
+        consider(t);
+    }
+    cur = best;
+    for (int bpc : {2, 4, 16}) {
+        rm_tune t = cur;
+        t.blocks_per_cu = bpc;
+        consider(t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r != RM_OK) return r;
+    b->tune = best;
+    if (int rr = batch_layout(b)) return rr;
+    if (int rr = batch_capture(b)) return rr;
+    if (chosen) *chosen = best;
+    return RM_OK;
+}
+
+int rm_batch_destroy(rm_batch* b) {
+    if (!b) return RM_OK;
+    rm_ctx* ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    batch_drop_graph(b);
+    if (b->d_views) (void)hipFree(b->d_views);
+    if (b->d_queue) (void)hipFree(b->d_queue);
+    delete b;
+    return RM_OK;
+}
+
+int rm_render_batch(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
+                    float* rgba, int32_t* hit) {
+    rm_batch* b = nullptr;
+    int r = rm_batch_create(ctx, cloud, views, nviews, nullptr, rgba, hit, &b);
+    if (r != RM_OK) return r;
+    r = rm_batch_launch(b);
+    int r2 = rm_batch_destroy(b);
+    return r != RM_OK ? r : r2;
+}
+
+// ---------------------------------------------------------------------------------------------
+// differentiable form
+// ---------------------------------------------------------------------------------------------
+int rm_render_diff(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov, int W, int H,
+                   float* rgb, float* depth, int32_t* hit) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (!c || !pose || !rgb || !depth || !hit)
+        return fail(ctx, RM_ERR_INVALID, "rm_render_diff: null input");
+    if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render_diff: empty image");
+    if (!(fov > 0.0f && fov < 3.14159f)) return fail(ctx, RM_ERR_INVALID, "rm_render_diff: bad fov");
+    if (int r = set_device(ctx)) return r;
+    int64_t npix = (int64_t)W * H;
+    if (int r = ensure_ws(ctx, align_up(sizeof(float4) * (size_t)npix))) return r;
+    float4* marched = reinterpret_cast<float4*>(ctx->ws);
+    DevView v = dev_view(pose, fov, W, H, 0, 0, 0);
+    DevCloud dc = dev_cloud(c);
+    timing_begin(ctx);
+    RM_HIP(ctx, rm::launch_render_plain(ctx->stream, dc, v, rm::MarchOut{marched, hit}, ctx->d_err));
+    RM_HIP(ctx, rm::launch_diff_forward(ctx->stream, dc, v, marched, hit, rgb, depth));
+    timing_end(ctx);
+    return RM_OK;
+}
+
+int rm_render_adjoint(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov, int W, int H,
+                      const int32_t* hit, const float* grad_rgb, const float* grad_depth,
+                      float* g_xyz, float* g_r, float* g_rgb) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (!c || !pose || !hit || !grad_rgb || !grad_depth || !g_xyz || !g_r || !g_rgb)
+        return fail(ctx, RM_ERR_INVALID, "rm_render_adjoint: null input");
+    if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render_adjoint: empty image");
+    if (int r = set_device(ctx)) return r;
+    int64_t npix = (int64_t)W * H;
+    if (npix > INT32_MAX) return fail(ctx, RM_ERR_UNSUPPORTED, "rm_render_adjoint: image too large");
+    int end_bit = 1;
+    while (end_bit < 32 && ((int64_t)1 << end_bit) <= c->n) ++end_bit;
+    size_t temp_bytes = 0;
+    RM_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (int32_t*)nullptr,
+                                                   (int32_t*)nullptr, (int32_t*)nullptr,
+                                                   (int32_t*)nullptr, (int)npix, 0, end_bit,
+                                                   ctx->stream));
+    size_t a = align_up(sizeof(int32_t) * (size_t)npix);
+    if (int r = ensure_ws(ctx, 4 * a + align_up(temp_bytes))) return r;
+    char* base = static_cast<char*>(ctx->ws);
+    int32_t* keys = reinterpret_cast<int32_t*>(base);
+    int32_t* vals = reinterpret_cast<int32_t*>(base + a);
+    int32_t* keys_s = reinterpret_cast<int32_t*>(base + 2 * a);
+    int32_t* vals_s = reinterpret_cast<int32_t*>(base + 3 * a);
+    void* temp = base + 4 * a;
+    DevView v = dev_view(pose, fov, W, H, 0, 0, 0);
+    timing_begin(ctx);
+    RM_HIP(ctx, rm::launch_adjoint_keys(ctx->stream, hit, npix, (int32_t)c->n, keys, vals));
+    RM_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_s, vals, vals_s,
+                                                   (int)npix, 0, end_bit, ctx->stream));
+    RM_HIP(ctx, rm::launch_adjoint_reduce(ctx->stream, dev_cloud(c), v, keys_s, vals_s, npix,
+                                          grad_rgb, grad_depth, g_xyz, g_r, g_rgb));
+    timing_end(ctx);
+    return RM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Internal declarations shared by the C-ABI layer (rm_api.hip) and the kernels
+// (rm_kernels.hip). Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rm_tree_host.h"
+
+namespace rm {
+
+// March constants of src/common.h:4-7 and src/ray_marching.h:17.
+constexpr float kZFar = 100.0f;
+constexpr float kZNear = 0.1f;
+constexpr int kMaxSteps = 128;
+constexpr float kMinDist = 0.001f;
+constexpr int kMarchK = 10;
+
+// Largest tree depth the traversals accept (stack bound); the host refuses deeper trees.
+constexpr int kMaxDepth = 60;
+constexpr int kWaveStack = 64;
+
+// Device view of a tree + cloud, passed by value to kernels.
+struct DevTree {
+    const RmNode* nodes;
+    const float4* pts;      // leaf order: x, y, z, original index bits
+    int64_t n;
+    int depth;
+};
+
+struct DevCloud {
+    DevTree tree;
+    const float4* xyzr;     // original order: x, y, z, radius
+    const float4* colors;   // original order: r, g, b, 1
+};
+
+// One view as the kernels see it.
+struct DevView {
+    float M[16];            // column-major camera-to-world
+    float pane;             // W / (2 tan(fov / 2)), computed on the host
+    int32_t W, H;
+    int32_t tiles_x;
+    int64_t pix_offset;     // first pixel in the concatenated outputs
+    int64_t tile_offset;    // first tile in the batch's tile numbering
+};
+
+struct MarchOut {
+    float4* rgba;
+    int32_t* hit;           // may be null
+};
+
+struct BatchParams {
+    DevCloud cloud;
+    const DevView* views;
+    int32_t nviews;
+    int32_t tile_w, tile_h;
+    int64_t total_tiles;
+    int64_t sample_stride;  // process tiles 0, s, 2s, ... (1 = all)
+    unsigned int* queue;    // tile counter
+    int* err;               // device error flag
+    MarchOut out;
+};
+
+// Kernel launchers (rm_kernels.hip). Each returns the hipError_t of the launch.
+hipError_t launch_knn(hipStream_t s, const DevTree& t, const float* q, int64_t m, int k,
+                      int32_t* idx, float* d2, int* err);
+hipError_t launch_radius(hipStream_t s, const DevTree& t, const float* q, int64_t m, float r2,
+                         int max_nn, int32_t* idx, float* d2, int32_t* counts, int* err);
+hipError_t launch_radii(hipStream_t s, const DevTree& t, float max_radius, float* r_unc,
+                        float* r_cl, int* err);
+hipError_t launch_make_cloud(hipStream_t s, const float* xyzw, const float* radii,
+                             const float* colors, int64_t n, float4* xyzr, float4* col);
+hipError_t launch_set_radii(hipStream_t s, const float* radii, int64_t n, float4* xyzr);
+hipError_t launch_set_colors(hipStream_t s, const float* colors, int64_t n, float4* col);
+hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& v, MarchOut o,
+                               int* err);
+hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
+                        int blocks);
+hipError_t launch_reset_queue(hipStream_t s, unsigned int* q);
+hipError_t launch_refit(hipStream_t s, RmNode* nodes, float4* pts, float4* xyzr,
+                        const float* xyz, int64_t n, const int32_t* level_order,
+                        const int32_t* level_start, int nlevels);
+hipError_t launch_diff_forward(hipStream_t s, const DevCloud& c, const DevView& v,
+                               const float4* rgba_marched, const int32_t* hit, float* rgb,
+                               float* depth);
+hipError_t launch_adjoint_keys(hipStream_t s, const int32_t* hit, int64_t npix, int32_t miss,
+                               int32_t* keys, int32_t* vals);
+hipError_t launch_adjoint_reduce(hipStream_t s, const DevCloud& c, const DevView& v,
+                                 const int32_t* keys_sorted, const int32_t* vals_sorted,
+                                 int64_t npix, const float* grad_rgb, const float* grad_depth,
+                                 float* g_xyz, float* g_r, float* g_rgb);
+
+}  // namespace rm

@@ -1,0 +1,45 @@
+"""CPU checks of the C ABI: the library loads and exports every symbol include/rm.h declares,
+and the host-side tree builder keeps its invariants (no GPU calls)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from renderer_ray_marching_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "rm.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int|float)\s+(rm_\w+)\(", src, re.M)))
+
+
+def test_header_declares_the_binding():
+    assert set(_declared()) == set(_abi.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _abi.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (rm_\w+)", out))
+    assert set(_declared()) <= exported
+
+
+def test_struct_layouts_match_header():
+    import ctypes as C
+    assert C.sizeof(_abi.rm_view) == 16 * 4 + 4 + 4 * 3 + 8
+    assert C.sizeof(_abi.rm_tune) == 6 * 4
+
+
+def test_ctx_create_without_device_fails_cleanly():
+    import ctypes as C
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    h = C.c_void_p()
+    assert _abi.lib().rm_ctx_create(0, C.byref(h)) == _abi.RM_ERR_INVALID

@@ -1,0 +1,314 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the parity reference.
+
+Every oracle function is checked against something that does not share its code path:
+brute force (itself checked against numpy on tiny inputs), closed forms (lattices, duplicates,
+two points, a single sphere), the tree rule's invariants, and the FLANN CPU kd-tree the
+reference vendors (committed fixtures in tests/golden/, regenerated from oracle/_ref).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from renderer_ray_marching_amd import scenes
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _cloud(n, seed):
+    return scenes.box_scene(n, seed=seed).xyzw()
+
+
+# ------------------------------------------------------------------------------------------
+# brute force: the definition, cross-checked with numpy
+# ------------------------------------------------------------------------------------------
+
+def test_brute_matches_numpy_definition():
+    rng = np.random.default_rng(5)
+    p = rng.random((300, 4)).astype(np.float32)
+    q = rng.random((40, 4)).astype(np.float32)
+    p[7, :3] = p[3, :3]          # an exact duplicate: tie broken by index
+    idx, d2 = ob.knn_brute(p, q, 7)
+    for j in range(q.shape[0]):
+        diff = p[:, :3] - q[j, :3]
+        dd = (diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1]) + diff[:, 2] * diff[:, 2]
+        order = np.lexsort((np.arange(p.shape[0]), dd))[:7]
+        assert (idx[j] == order).all()
+        assert (d2[j] == dd[order]).all()
+
+
+def test_brute_padding_when_fewer_points_than_k():
+    p = np.array([[0, 0, 0, 1], [1, 0, 0, 1]], np.float32)
+    idx, d2 = ob.knn_brute(p, p, 4)
+    assert (idx[:, 2:] == -1).all() and np.isinf(d2[:, 2:]).all()
+    assert (idx[0, :2] == [0, 1]).all() and (d2[0, :2] == [0.0, 1.0]).all()
+
+
+# ------------------------------------------------------------------------------------------
+# reference tree rule
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("leaf", [1, 8, 64])
+def test_tree_rule_invariants(leaf):
+    p = _cloud(5000, 3)
+    t = ob.RefTree(p, leaf)
+    e = t.export()
+    order = e["order"]
+    assert sorted(order.tolist()) == list(range(p.shape[0]))
+    nn = t.num_nodes
+    for i in range(nn):
+        b, en = e["begin"][i], e["end"][i]
+        pts = p[order[b:en], :3]
+        assert (e["lo"][i] == pts.min(0)).all() and (e["hi"][i] == pts.max(0)).all()
+        c = e["child1"][i]
+        if c == -1:
+            assert en - b <= leaf or (pts == pts[0]).all()
+            continue
+        assert e["parent"][c] == i and e["parent"][c + 1] == i
+        assert e["begin"][c] == b and e["end"][c + 1] == en and e["end"][c] == e["begin"][c + 1]
+        ext = e["hi"][i] - e["lo"][i]
+        dim = 0
+        for d in (1, 2):
+            if ext[d] > ext[dim]:
+                dim = d
+        split = np.float32((e["hi"][i][dim] + e["lo"][i][dim]) * np.float32(0.5))
+        left = p[order[e["begin"][c]:e["end"][c]], dim]
+        right = p[order[e["begin"][c + 1]:e["end"][c + 1]], dim]
+        assert (left <= split).all() and (right > split).all()
+
+
+def test_tree_all_equal_points_stay_a_leaf():
+    p = np.tile(np.array([[1.5, -2.0, 0.25, 1.0]], np.float32), (200, 1))
+    t = ob.RefTree(p, 64)
+    assert t.num_nodes == 1
+    ru, rc, _ = t.radii(0.1)
+    assert (ru == 0).all() and (rc == 0).all()
+
+
+# ------------------------------------------------------------------------------------------
+# kNN by the reference traversal == brute force (exact under the (d2, index) order)
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("k", [1, 2, 10, 32])
+@pytest.mark.parametrize("seed", [1, 7])
+def test_oracle_knn_equals_brute_force(k, seed):
+    p = _cloud(20000, seed)
+    rng = np.random.default_rng(seed)
+    q = np.concatenate([p[:1500], p[rng.integers(0, p.shape[0], 500)] +
+                        rng.normal(0, 0.05, (500, 4)).astype(np.float32),
+                        rng.uniform(-1, 5, (500, 4)).astype(np.float32)])
+    t = ob.RefTree(p, 64)
+    i1, d1, cnt = t.knn(q, k)
+    i2, d2 = ob.knn_brute(p, q, k)
+    assert (i1 == i2).all() and (d1 == d2).all()
+    assert cnt.q == q.shape[0] and cnt.c >= q.shape[0] * min(k, 1)
+
+
+def test_oracle_knn_lattice_ties_broken_by_index():
+    # queries at cell centres of a lattice are equidistant from 8 corners: exact ties
+    h = np.float32(0.25)
+    g = np.arange(6, dtype=np.float32) * h
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel(), np.ones(X.size, np.float32)], 1)
+    rng = np.random.default_rng(0)
+    p = p[rng.permutation(p.shape[0])].astype(np.float32)
+    q = p[:50].copy()
+    q[:, :3] += h / 2
+    t = ob.RefTree(p, 8)
+    i1, d1, _ = t.knn(q, 10)
+    i2, d2 = ob.knn_brute(p, q, 10)
+    assert (i1 == i2).all() and (d1 == d2).all()
+    # the 8 corners tie at 3 (h/2)^2; ascending index among them
+    for j in range(q.shape[0]):
+        eq = d1[j] == d1[j, 0]
+        assert (np.diff(i1[j][eq]) > 0).all()
+
+
+# ------------------------------------------------------------------------------------------
+# radius rule (main.cu:99-149)
+# ------------------------------------------------------------------------------------------
+
+def test_radii_lattice_closed_form():
+    h = np.float32(0.125)
+    g = np.arange(12, dtype=np.float32) * h
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel(), np.ones(X.size, np.float32)], 1)
+    t = ob.RefTree(p, 64)
+    ru, rc, _ = t.radii(0.1)
+    assert (ru == h).all()
+    assert (rc == np.float32(0.1)).all()
+    ru2, rc2, _ = t.radii(0.0)          # max_radius <= 0 disables the clamp
+    assert (rc2 == h).all()
+
+
+def test_radii_duplicates_and_pairs():
+    p = np.array([[0, 0, 0, 1], [0, 0, 0, 1], [3, 4, 0, 1], [3, 4, 12, 1]], np.float32)
+    t = ob.RefTree(p, 64)
+    ru, rc, _ = t.radii(100.0)
+    assert ru[0] == 0 and ru[1] == 0
+    assert ru[2] == np.float32(5.0)     # to (0,0,0)
+    assert ru[3] == np.float32(12.0)    # to (3,4,0)
+
+
+def test_radii_equal_brute_force():
+    p = _cloud(30000, 11)
+    t = ob.RefTree(p, 64)
+    ru, rc, cnt = t.radii(0.01)
+    _, d2 = ob.knn_brute(p, p, 2)
+    assert (ru == np.sqrt(d2[:, 1])).all()
+    assert (rc == np.minimum(ru, np.float32(0.01))).all()
+    assert cnt.q == p.shape[0]
+
+
+def test_radii_single_point_is_infinite():
+    t = ob.RefTree(np.array([[1, 2, 3, 1]], np.float32), 64)
+    ru, rc, _ = t.radii(0.1)
+    assert np.isinf(ru[0]) and rc[0] == np.float32(0.1)
+
+
+# ------------------------------------------------------------------------------------------
+# FLANN CPU kd-tree (reference's vendored FLANN 1.9.1), via committed fixtures
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["flann_box10k_k2", "flann_box10k_k10"])
+def test_oracle_against_flann_fixture(name):
+    """FLANN's 'exact' CPU search (kdtree_index.h:628-664) re-adds the split distance of an
+    axis already used by a parent, so its bound can overshoot and it occasionally misses a true
+    neighbour. The oracle must agree with it wherever FLANN returns the exact set and must never
+    be worse (its k-th distance <= FLANN's)."""
+    f = np.load(os.path.join(GOLDEN, name + ".npz"))
+    p, q, k = f["points"], f["queries"], int(f["k"])
+    idx_f, d2_f = f["idx"], f["d2"]
+    t = ob.RefTree(p, 64)
+    idx, d2, _ = t.knn(q, k)
+    # rank by rank, the exact list is never worse than any k points FLANN can return
+    assert (d2 <= d2_f).all()
+    # FLANN's distances are the true distances of the points it names
+    diff = p[idx_f, :3] - q[:, None, :3]
+    dd = (diff[..., 0] * diff[..., 0] + diff[..., 1] * diff[..., 1]) + diff[..., 2] * diff[..., 2]
+    assert (dd == d2_f).all()
+    same = (d2 == d2_f).all(1)
+    assert same.mean() > (0.98 if k == 2 else 0.9)
+    # where the distance lists agree, the index sets agree too (up to ties' order)
+    for j in np.nonzero(same)[0][:2000]:
+        assert set(idx[j].tolist()) == set(idx_f[j].tolist()) or (np.diff(d2[j]) == 0).any()
+
+
+@pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built (needs the reference)")
+def test_flann_live_cross_check():
+    """FLANN's tree picks split axes at random (rand()), so its misses vary run to run; the
+    invariants above hold for every run."""
+    p = _cloud(20000, 9)
+    rng = np.random.default_rng(9)
+    q = (p[rng.integers(0, p.shape[0], 3000)] + rng.normal(0, 0.02, (3000, 4))).astype(np.float32)
+    t = ob.RefTree(p, 64)
+    for k in (2, 10):
+        idx_f, d2_f = ob.flann_knn(p, q, k)
+        idx, d2, _ = t.knn(q, k)
+        assert (d2 <= d2_f).all()
+        assert (d2 == d2_f).all(1).mean() > (0.98 if k == 2 else 0.9)
+
+
+# ------------------------------------------------------------------------------------------
+# march (src/ray_marching.cu:63-91, 202-228, 260-292)
+# ------------------------------------------------------------------------------------------
+
+def _identity_view(W=64, H=48, f=40.0):
+    pose = np.eye(4, dtype=np.float32)           # camera at origin looking down -z
+    return pose, scenes.fov_radians(W, f), W, H
+
+
+def _one_sphere(c, r):
+    p = np.array([[c[0], c[1], c[2], 1.0]], np.float32)
+    t = ob.RefTree(p, 64)
+    return t, np.array([r], np.float32), np.array([[0.2, 0.4, 0.6, 1.0]], np.float32)
+
+
+def test_march_single_sphere_depth_closed_form():
+    pose, fov, W, H = _identity_view()
+    c, r = np.array([0.3, -0.2, -4.0]), 0.7
+    t, radii, col = _one_sphere(c, r)
+    rgba, hit, cnt = t.render(radii, col, pose.T.ravel(), fov, W, H)
+    pane = ob.pane_distance(W, fov)
+    nhit = 0
+    for lin in range(W * H):
+        x, y = lin % W, lin // W
+        d = np.array([x - W / 2, y - H / 2, -pane], np.float64)
+        d /= np.linalg.norm(d)
+        b = d @ c
+        disc = b * b - (c @ c - r * r)
+        if disc > 0.1 * r * r:      # well inside the silhouette: tracing converges fast
+            ts = b - math.sqrt(disc)
+            assert hit[lin] == 0
+            depth = rgba[lin, 3]
+            # never past the surface (the single-sphere SDF is exact), and within a few MIN_DIST
+            assert depth <= ts + 1e-5 and ts - depth < 0.005
+            assert (rgba[lin, :3] == col[0, :3]).all()
+            nhit += 1
+        elif disc < -1e-3:          # clearly missed
+            assert hit[lin] == -1 and (rgba[lin] == 0).all()
+    assert nhit > 100
+
+
+def test_march_centre_ray_is_minus_z():
+    pose, fov, W, H = _identity_view(64, 48, 40.0)
+    # a sphere straight ahead on the optical axis at distance 2 with radius 0.5
+    t, radii, col = _one_sphere((0.0, 0.0, -2.0), 0.5)
+    lin = (H // 2) * W + W // 2
+    rgba, hit, _ = t.render(radii, col, pose.T.ravel(), fov, W, H, pix=[lin])
+    assert hit[0] == 0
+    assert 1.5 - 0.001 <= rgba[0, 3] <= 1.5
+
+
+def test_march_start_inside_sphere_hits_at_step_zero():
+    pose, fov, W, H = _identity_view()
+    t, radii, col = _one_sphere((0.0, 0.0, 0.0), 1.0)
+    rgba, hit, cnt = t.render(radii, col, pose.T.ravel(), fov, W, H)
+    assert (hit == 0).all()
+    assert cnt.s == W * H                        # one step per ray
+    # depth = near + (|p| - r) with p on the near plane
+    assert (rgba[:, 3] < 0.1).all() and (rgba[:, 3] > -1.0).all()
+
+
+def test_march_tiny_scene_equals_brute_sdf():
+    """With <= 10 points kNN-10 is the whole cloud, so the SDF is min(|p - c| - r) over all
+    points (the brute-force SDF of shaders/compute.glsl:33-48): re-march with a numpy
+    brute-force SDF in float32 and compare hit and depth exactly."""
+    rng = np.random.default_rng(3)
+    pts = np.concatenate([rng.uniform([-1, -1, -5], [1, 1, -3], (8, 3))]).astype(np.float32)
+    radii = rng.uniform(0.2, 0.5, 8).astype(np.float32)
+    p4 = np.concatenate([pts, np.ones((8, 1), np.float32)], 1)
+    t = ob.RefTree(p4, 64)
+    col = np.ones((8, 4), np.float32)
+    pose, fov, W, H = _identity_view(48, 32, 30.0)
+    rgba, hit, _ = t.render(radii, col, pose.T.ravel(), fov, W, H)
+    pane = np.float32(ob.pane_distance(W, fov))
+    f32 = np.float32
+    for lin in range(0, W * H, 7):
+        x, y = lin % W, lin // W
+        u, v = f32(x) - f32(W / 2), f32(y) - f32(H / 2)
+        z = -pane
+        inv = f32(1.0) / np.sqrt(f32(u * u + v * v) + f32(z * z), dtype=np.float32)
+        d = np.array([u * inv, v * inv, z * inv], np.float32)
+        cos = -d[2]
+        tt, far = f32(0.1) / cos, f32(100.0) / cos
+        res = -1
+        for _ in range(128):
+            pnt = d * tt
+            diff = pts - pnt
+            dd = (diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1]) + diff[:, 2] * diff[:, 2]
+            order = np.lexsort((np.arange(8), dd))
+            s_all = np.sqrt(dd[order]) - radii[order]
+            j = int(np.argmin(s_all))
+            s = s_all[j]
+            tt = f32(tt + s)
+            if s < f32(0.001):
+                res = int(order[j])
+                break
+            if tt > far:
+                break
+        assert hit[lin] == res
+        if res >= 0:
+            assert rgba[lin, 3] == tt
