@@ -37,6 +37,10 @@ def main():
     ap.add_argument("--workload", default="room10m", choices=["room10m", "room2m", "box100k"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch tune")
+    ap.add_argument("--tune", default=None,
+                    help="JSON rm_tune to use instead of autotuning (e.g. from a previous run)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (HBM counter calibration)")
     ap.add_argument("--plain-views", type=int, default=0,
                     help="time the plain path on the first N views only (0 = all)")
     a = ap.parse_args()
@@ -58,6 +62,12 @@ def main():
     log(f"scene {a.workload}: {cloud.n} points, {len(views)} views ({time.time() - t0:.1f}s)")
     rays = sum(v.width * v.height for v in views)
 
+    if a.copy_calibration:
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)          # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
     ctx = api.Context(0)
     p = torch.from_numpy(cloud.xyzw()).cuda()
     col = torch.from_numpy(cloud.colors4()).cuda()
@@ -103,9 +113,10 @@ def main():
     log(f"plain: {nplain} views, {plain_s:.3f}s, {plain_mrays:.1f} Mrays/s")
 
     # ---- batched path: persistent kernel, tile queue, HIP graph ----
-    batch = api.Batch(marcher, views)
+    fixed = json.loads(a.tune) if a.tune else None
+    batch = api.Batch(marcher, views, tune=fixed)
     tune = batch.tune()
-    if not a.no_tune:
+    if not a.no_tune and fixed is None:
         tt = time.time()
         tune = batch.autotune(sample_stride=16)
         log(f"autotune: {tune} ({time.time() - tt:.1f}s)")

@@ -29,7 +29,7 @@ class rm_view(C.Structure):
 class rm_tune(C.Structure):
     _fields_ = [("tile_w", C.c_int32), ("tile_h", C.c_int32), ("mode", C.c_int32),
                 ("warm", C.c_int32), ("waves_per_block", C.c_int32),
-                ("blocks_per_cu", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("refill", C.c_int32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -58,7 +59,7 @@ struct rm_batch {
     int64_t total_tiles = 0;
     int64_t total_pixels = 0;
     rm_tune tune{};
-    unsigned int* d_queue = nullptr;
+    unsigned long long* d_queue = nullptr;
     float* rgba = nullptr;
     int32_t* hit = nullptr;
     hipGraph_t graph = nullptr;
@@ -457,6 +458,7 @@ rm_tune default_tune() {
     t.warm = 1;
     t.waves_per_block = 2;
     t.blocks_per_cu = 0;
+    t.refill = 8;
     return t;
 }
 
@@ -464,7 +466,7 @@ bool tune_valid(const rm_tune& t) {
     return t.tile_w > 0 && t.tile_h > 0 && t.tile_w * t.tile_h == 64 && (t.mode == 0 || t.mode == 1) &&
            (t.warm == 0 || t.warm == 1) &&
            (t.waves_per_block == 1 || t.waves_per_block == 2 || t.waves_per_block == 4) &&
-           t.blocks_per_cu >= 0 && t.blocks_per_cu <= 64;
+           t.blocks_per_cu >= 0 && t.blocks_per_cu <= 64 && t.refill >= 0 && t.refill <= 64;
 }
 
 // Lay out the views' tiles for a tune and upload the view table.
@@ -512,7 +514,7 @@ int batch_enqueue(rm_batch* b, hipStream_t s, int64_t sample_stride) {
     rm_ctx* ctx = b->ctx;
     RM_HIP(ctx, rm::launch_reset_queue(s, b->d_queue));
     RM_HIP(ctx, rm::launch_batch(s, batch_params(b, sample_stride), b->tune.mode, b->tune.warm,
-                                 b->tune.waves_per_block, b->blocks));
+                                 b->tune.waves_per_block, b->blocks, b->tune.refill));
     return RM_OK;
 }
 
@@ -569,7 +571,7 @@ int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
     b->rgba = rgba;
     b->hit = hit;
     if (hipMalloc(&b->d_views, sizeof(DevView) * nviews) != hipSuccess ||
-        hipMalloc(&b->d_queue, sizeof(unsigned int)) != hipSuccess) {
+        hipMalloc(&b->d_queue, sizeof(unsigned long long)) != hipSuccess) {
         (void)hipGetLastError();
         rm_batch_destroy(b);
         return fail(ctx, RM_ERR_NOMEM, "rm_batch_create: device allocation failed");
@@ -626,26 +628,40 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
     };
     rm_tune best = b->tune;
     float best_ms = 1e30f;
+    const char* env = std::getenv("RM_TUNE_VERBOSE");
+    const bool verbose = env && env[0] == '1';
     int r = RM_OK;
     auto consider = [&](rm_tune t) {
         if (r != RM_OK || !tune_valid(t)) return;
         if (t.warm && b->cloud->n < rm::kMarchK) return;
         float ms;
         r = measure(t, ms);
+        if (r == RM_OK && verbose)
+            std::fprintf(stderr,
+                         "[rm autotune] tile %dx%d mode %d warm %d wpb %d bpc %d refill %d: "
+                         "%.3f ms\n",
+                         t.tile_w, t.tile_h, t.mode, t.warm, t.waves_per_block, t.blocks_per_cu,
+                         t.refill, ms);
         if (r == RM_OK && ms < best_ms) {
             best_ms = ms;
             best = t;
         }
     };
-    // coordinate descent: traversal mode x warm start, then tile shape, then block shape
+    // coordinate descent: traversal mode x warm start x schedule, then tile shape, then
+    // block shape, then the refill threshold
     rm_tune base = default_tune();
+    // (refill is only tried with per-lane traversal: refilled lanes mix screen regions, which
+    // costs a packet traversal far more than it saves)
     for (int mode = 0; mode < 2; ++mode)
-        for (int warm = 0; warm < 2; ++warm) {
-            rm_tune t = base;
-            t.mode = mode;
-            t.warm = warm;
-            consider(t);
-        }
+        for (int warm = 0; warm < 2; ++warm)
+            for (int refill : {0, 8}) {
+                if (mode == 0 && refill > 0) continue;
+                rm_tune t = base;
+                t.mode = mode;
+                t.warm = warm;
+                t.refill = refill;
+                consider(t);
+            }
     const int shapes[][2] = {{16, 4}, {4, 16}, {32, 2}};
     rm_tune cur = best;
     for (auto& sh : shapes) {
@@ -666,6 +682,13 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         t.blocks_per_cu = bpc;
         consider(t);
     }
+    cur = best;
+    if (cur.refill > 0)
+        for (int rf : {1, 16, 32}) {
+            rm_tune t = cur;
+            t.refill = rf;
+            consider(t);
+        }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (r != RM_OK) return r;

@@ -56,7 +56,7 @@ struct BatchParams {
     int32_t tile_w, tile_h;
     int64_t total_tiles;
     int64_t sample_stride;  // process tiles 0, s, 2s, ... (1 = all)
-    unsigned int* queue;    // tile counter
+    unsigned long long* queue;  // work counter (tiles, or pixels with refill)
     int* err;               // device error flag
     MarchOut out;
 };
@@ -75,8 +75,8 @@ hipError_t launch_set_colors(hipStream_t s, const float* colors, int64_t n, floa
 hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& v, MarchOut o,
                                int* err);
 hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
-                        int blocks);
-hipError_t launch_reset_queue(hipStream_t s, unsigned int* q);
+                        int blocks, int refill);
+hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q);
 hipError_t launch_refit(hipStream_t s, RmNode* nodes, float4* pts, float4* xyzr,
                         const float* xyz, int64_t n, const int32_t* level_order,
                         const int32_t* level_start, int nlevels);

@@ -501,9 +501,10 @@ __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxs
     CFloat* vf = (CFloat*)p.views;
     constexpr int VSTRIDE = sizeof(DevView) / 4;
     while (true) {
-        unsigned int qv = 0;
-        if (lane == 0) qv = atomicAdd(p.queue, 1u);
-        qv = __builtin_amdgcn_readfirstlane(qv);
+        unsigned long long qv = 0;
+        if (lane == 0) qv = atomicAdd(p.queue, 1ull);
+        qv = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(qv >> 32)) << 32) |
+             (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qv);
         int64_t tile = (int64_t)qv * p.sample_stride;
         if (tile >= p.total_tiles) break;
         // view lookup: last view whose tile_offset <= tile
@@ -544,7 +545,102 @@ __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxs
     }
 }
 
-__global__ void reset_queue_kernel(unsigned int* q) { *q = 0u; }
+// Batched path with per-lane refill: a lane whose ray has ended takes the next pixel of the
+// shared queue (pixels numbered tile by tile, so lanes refilled together stay close on screen),
+// and the wavefront keeps stepping until the queue is drained. One queue claim serves every
+// lane that needs work (ballot + prefix count), and claims wait until at least `refill_min`
+// lanes are idle so the counter sees few atomics. A wavefront step advances every live ray by
+// one march step (one kNN-10 query each), in packet (MODE 0) or per-lane (MODE 1) traversal.
+template <int MODE, bool WARM, int WPB>
+__global__ __launch_bounds__(WPB * 64) void batch_refill_kernel(BatchParams p, int maxsp,
+                                                                int refill_min) {
+    __shared__ int wstk[WPB][kWaveStack];
+    extern __shared__ int lstk[];
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int64_t ntiles = (p.total_tiles + p.sample_stride - 1) / p.sample_stride;
+    const int64_t nwork = ntiles * 64;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    bool has = false;
+    bool drained = false;          // wave-uniform
+    Ray r = {0.f, 0.f, 0.f, 0.f, 0.f, -1.f, 0.f, 0.f};
+    float t = 0.0f;
+    int steps = 0;
+    int64_t out = 0;
+    TopK<kMarchK> tk;
+    tk.init(kPadKey);
+    while (true) {
+        uint64_t needm = __ballot(!has);
+        int nneed = __popcll(needm);
+        if (!drained && (nneed >= refill_min || nneed == 64)) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(p.queue, (unsigned long long)nneed);
+            base = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32))
+                    << 32) | (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
+            if ((int64_t)base + nneed >= nwork) drained = true;
+            if (!has) {
+                int64_t wi = (int64_t)base + __popcll(needm & lt_mask);
+                if (wi < nwork) {
+                    int64_t tile = (wi >> 6) * p.sample_stride;
+                    int sub = (int)(wi & 63);
+                    int lo = 0, hi = p.nviews - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (p.views[mid].tile_offset <= tile)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const DevView* V = p.views + lo;
+                    const int W = V->W, H = V->H, tiles_x = V->tiles_x;
+                    int64_t lt = tile - V->tile_offset;
+                    int x = (int)(lt % tiles_x) * p.tile_w + sub % p.tile_w;
+                    int y = (int)(lt / tiles_x) * p.tile_h + sub / p.tile_w;
+                    if (x < W && y < H) {
+                        r = make_ray([&](int i) { return V->M[i]; }, V->pane, W, H, x, y);
+                        t = r.tnear;
+                        steps = 0;
+                        out = V->pix_offset + (int64_t)y * W + x;
+                        has = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(has) == 0) {
+            if (drained) break;
+            continue;
+        }
+        float px = r.ox + r.dx * t;
+        float py = r.oy + r.dy * t;
+        float pz = r.oz + r.dz * t;
+        if (!has)
+            tk.init(0ull);
+        else if (WARM && steps > 0)
+            warm_init(p.cloud, tk, px, py, pz);
+        else
+            tk.init(kPadKey);
+        if (MODE == 0) {
+            knn_packet<kMarchK>(p.cloud.tree, px, py, pz, has, tk, wstk[w], p.err);
+        } else if (has) {
+            knn_lane<kMarchK>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x], WPB * 64, maxsp,
+                              p.err);
+        }
+        if (has) {
+            float s;
+            uint32_t id;
+            sdf10(p.cloud, tk, s, id);
+            t = t + s;
+            ++steps;
+            bool hitnow = s < kMinDist;
+            if (hitnow || t > r.tfar || steps >= kMaxSteps) {
+                write_pixel(p.cloud, p.out, out, hitnow ? (int)id : -1, hitnow ? t : 0.0f);
+                has = false;
+            }
+        }
+    }
+}
+
+__global__ void reset_queue_kernel(unsigned long long* q) { *q = 0ull; }
 
 // ------------------------------------------------------------------------------------------
 // Cloud set-up and refit
@@ -821,18 +917,23 @@ hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& 
 }
 
 template <int MODE, bool WARM, int WPB>
-static hipError_t launch_batch_t(hipStream_t s, const BatchParams& p, int blocks) {
+static hipError_t launch_batch_t(hipStream_t s, const BatchParams& p, int blocks, int refill) {
     int maxsp = lane_stack_depth(p.cloud.tree);
     size_t shmem = MODE == 1 ? (size_t)maxsp * WPB * 64 * sizeof(int) : 0;
-    hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB>), dim3(blocks), dim3(WPB * 64), shmem, s,
-                       p, maxsp);
+    if (refill > 0)
+        hipLaunchKernelGGL((batch_refill_kernel<MODE, WARM, WPB>), dim3(blocks), dim3(WPB * 64),
+                           shmem, s, p, maxsp, refill);
+    else
+        hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB>), dim3(blocks), dim3(WPB * 64), shmem,
+                           s, p, maxsp);
     return hipGetLastError();
 }
 
 hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
-                        int blocks) {
+                        int blocks, int refill) {
 #define RM_B(M, W, P) \
-    if (mode == M && (warm != 0) == W && wpb == P) return launch_batch_t<M, W, P>(s, p, blocks);
+    if (mode == M && (warm != 0) == W && wpb == P) \
+        return launch_batch_t<M, W, P>(s, p, blocks, refill);
     RM_B(0, false, 1) RM_B(0, false, 2) RM_B(0, false, 4)
     RM_B(0, true, 1) RM_B(0, true, 2) RM_B(0, true, 4)
     RM_B(1, false, 1) RM_B(1, false, 2) RM_B(1, false, 4)
@@ -841,7 +942,7 @@ hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm,
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_reset_queue(hipStream_t s, unsigned int* q) {
+hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q) {
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, q);
     return hipGetLastError();
 }

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     import ctypes as C
     assert C.sizeof(_abi.rm_view) == 16 * 4 + 4 + 4 * 3 + 8
-    assert C.sizeof(_abi.rm_tune) == 6 * 4
+    assert C.sizeof(_abi.rm_tune) == 7 * 4
 
 
 def test_ctx_create_without_device_fails_cleanly():
