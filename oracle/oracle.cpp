@@ -485,13 +485,19 @@ double orc_loss_grad_fixed(const double* xyz, const double* r, const double* rgb
         int x = (int)(lin % W), y = (int)(lin / W);
         Ray ray = make_ray(pose, pane, W, H, x, y);
         double o[3] = {ray.ox, ray.oy, ray.oz}, d[3] = {ray.dx, ray.dy, ray.dz};
-        double m[3], b = 0.0, mm = 0.0;
+        // m = o - c, b = d.m, h = m - b d (centre-to-ray offset), disc = r^2 - |h|^2,
+        // entry depth t = -b - sqrt(disc)
+        double m[3], b = 0.0;
         for (int a = 0; a < 3; ++a) {
             m[a] = o[a] - xyz[3 * h + a];
             b += d[a] * m[a];
-            mm += m[a] * m[a];
         }
-        double disc = b * b - (mm - r[h] * r[h]);
+        double hv[3], h2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            hv[a] = m[a] - b * d[a];
+            h2 += hv[a] * hv[a];
+        }
+        double disc = r[h] * r[h] - h2;
         // colour term
         for (int a = 0; a < 3; ++a) {
             double e = rgb[3 * h + a] - (double)target_rgb[3 * lin + a];
@@ -504,8 +510,8 @@ double orc_loss_grad_fixed(const double* xyz, const double* r, const double* rgb
         double e = depth - (double)target_depth[lin];
         loss += w_depth * std::fabs(e);
         double gd = w_depth * (e > 0 ? 1.0 : (e < 0 ? -1.0 : 0.0));
-        // dt/dc = d - (m - b d)/sqrt(disc),  dt/dr = -r/sqrt(disc)
-        for (int a = 0; a < 3; ++a) g_xyz[3 * h + a] += gd * (d[a] - (m[a] - b * d[a]) / sq);
+        // dt/dc = d - h / sqrt(disc),  dt/dr = -r / sqrt(disc)
+        for (int a = 0; a < 3; ++a) g_xyz[3 * h + a] += gd * (d[a] - hv[a] / sq);
         g_r[h] += gd * (-r[h] / sq);
     }
     return loss;
