@@ -41,6 +41,10 @@ def main():
                     help="JSON rm_tune to use instead of autotuning (e.g. from a previous run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (HBM counter calibration)")
+    ap.add_argument("--no-diff", action="store_true",
+                    help="skip the differentiable-form leg (fwd and fwd+bwd ms/view)")
+    ap.add_argument("--diff-steps", type=int, default=20,
+                    help="Adam steps of the differentiable leg")
     ap.add_argument("--plain-views", type=int, default=0,
                     help="time the plain path on the first N views only (0 = all)")
     a = ap.parse_args()
@@ -176,6 +180,10 @@ def main():
                           "compulsory_frac": round(16 * cloud.n / (radii_ms * 1e-3) / 1e9 /
                                                    HBM_PEAK_GBS, 5)}
 
+    del batch, plain_out, marcher, tree, p, col
+    torch.cuda.empty_cache()
+    diff = None if a.no_diff else diff_leg(api, scenes, torch, a.diff_steps)
+
     # ---- CPU baseline: the oracle's march on a bounded pixel sample, all host cores ----
     cpu = None
     if not a.no_cpu:
@@ -208,9 +216,64 @@ def main():
         "radii_ms": round(radii_ms, 4),
         "radii_roofline": radii_roof,
         "roofline": roofline,
+        "diff": diff,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def diff_leg(api, scenes, torch, steps):
+    """Differentiable form on 1M box-scene points, 16 views at 512x512: forward and
+    forward+backward device time per view, and an Adam refinement from the perturbed cloud
+    toward targets rendered from the clean one."""
+    from renderer_ray_marching_amd import diff as D
+    cloud, views, max_r = scenes.workload_box_1m_diff()
+    ctx = api.Context(0)
+    p = torch.from_numpy(cloud.xyzw()).cuda()
+    tree = api.KDTreeFlann(ctx)
+    tree.Build(p)
+    _, rc = tree.radii(max_r)
+    xyz_c = torch.from_numpy(cloud.xyz).cuda()
+    rgb_c = torch.from_numpy(cloud.colors4()[:, :3].copy()).cuda()
+    Rt = D.DiffRenderer(xyz_c, rc, rgb_c, views, ctx=ctx)
+    timg, tdep, thit = Rt.render(xyz_c, rc, rgb_c)
+    tdep = torch.where(thit >= 0, tdep, torch.zeros_like(tdep))
+    x0, r0, c0 = D.perturbed_cloud(cloud.xyz, rc.cpu().numpy(), cloud.rgb, seed=104)
+    x0, r0, c0 = (torch.from_numpy(v).cuda() for v in (x0, r0, c0))
+    R = D.DiffRenderer(x0, r0, c0, views, ctx=ctx)
+    x = x0.clone().requires_grad_(True)
+    r = r0.clone().requires_grad_(True)
+    c = c0.clone().requires_grad_(True)
+
+    def loss_fn(img, depth, hit):
+        m = ((hit >= 0) & (tdep > 0)).float()
+        return (((img - timg) ** 2).sum(-1) * m).sum() + ((depth - tdep).abs() * m).sum()
+
+    for _ in range(2):                                   # warm-up
+        loss_fn(*R.render(x, r, c)).backward()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        with torch.no_grad():
+            R.render(x, r, c)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss_fn(*R.render(x, r, c)).backward()
+    torch.cuda.synchronize()
+    fb = (time.perf_counter() - t0) / reps
+    nv = len(views)
+    log(f"diff: fwd {fwd / nv * 1e3:.2f} ms/view, fwd+bwd {fb / nv * 1e3:.2f} ms/view")
+    _, _, _, losses = D.refine(R, x0, r0, c0, timg, tdep, steps=steps)
+    log(f"diff: Adam {steps} steps, loss {losses[0]:.1f} -> {losses[-1]:.1f}")
+    return {"workload": f"box1m: {cloud.n} points, {nv} views 512x512",
+            "fwd_ms_per_view": round(fwd / nv * 1e3, 3),
+            "fwd_bwd_ms_per_view": round(fb / nv * 1e3, 3),
+            "bwd_over_fwd": round(fb / fwd, 3),
+            "adam_steps": steps, "loss_first": round(losses[0], 2),
+            "loss_last": round(losses[-1], 2)}
 
 
 def cpu_baseline(cloud, views, radii):

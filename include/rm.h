@@ -126,13 +126,14 @@ typedef struct rm_view {
 
 /* Launch parameters picked by the autotuner. */
 typedef struct rm_tune {
-    int32_t tile_w, tile_h;   /* pixel tile per wavefront packet, tile_w * tile_h == 64 */
+    int32_t tile_w, tile_h;   /* pixels per claimed tile: tile_w * tile_h, a multiple of 64
+                                 (<= 4096); a wavefront keeps 64 rays of its tile in flight */
     int32_t mode;             /* 0 = packet traversal (wave-uniform), 1 = per-lane traversal */
     int32_t warm;             /* 1 = warm-start bound from the previous step's neighbours */
     int32_t waves_per_block;  /* 1, 2 or 4 */
     int32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu (0 = 16 / waves_per_block) */
-    int32_t refill;           /* 0 = a wavefront marches whole tiles; k > 0 = lanes whose ray ended
-                                 take new pixels once at least k lanes are idle */
+    int32_t refill;           /* lanes whose ray ended take the tile's next pixels once at
+                                 least this many (1..64) are idle */
 } rm_tune;
 
 /* A batch of views marched by one persistent kernel over a shared tile queue, launched as a

@@ -57,6 +57,9 @@ def lib():
         L.orc_pane_distance.argtypes = [C.c_int, C.c_float]
         L.orc_render.argtypes = [C.c_void_p, _f, _f, _f, C.c_float, C.c_int, C.c_int,
                                  C.c_void_p, C.c_int64, _f, _i, C.POINTER(Counters), C.c_int]
+        L.orc_render_steps.argtypes = [C.c_void_p, _f, _f, _f, C.c_float, C.c_int, C.c_int,
+                                       C.c_void_p, C.c_int64, _f, _i, _i,
+                                       C.POINTER(Counters), C.c_int]
         L.orc_loss_grad_fixed.restype = C.c_double
         L.orc_loss_grad_fixed.argtypes = [_d, _d, _d, C.c_int64, _f, C.c_float, C.c_int,
                                           C.c_int, _i, _f, _f, C.c_double, C.c_double, _d, _d,
@@ -140,7 +143,8 @@ class RefTree:
         lib().orc_radii(self.h, max_radius, ru, rc, C.byref(cnt), nthreads)
         return ru, rc, cnt
 
-    def render(self, radii, colors4, pose_colmajor, fov, W, H, pix=None, nthreads=0):
+    def render(self, radii, colors4, pose_colmajor, fov, W, H, pix=None, nthreads=0,
+               steps=False):
         pane = lib().orc_pane_distance(W, fov)
         radii = np.ascontiguousarray(radii, np.float32)
         colors4 = np.ascontiguousarray(colors4, np.float32)
@@ -155,6 +159,11 @@ class RefTree:
         rgba = np.empty((npix, 4), np.float32)
         hit = np.empty(npix, np.int32)
         cnt = Counters()
+        if steps:
+            st = np.empty(npix, np.int32)
+            lib().orc_render_steps(self.h, radii, colors4, pose, pane, W, H, pptr, npix, rgba,
+                                   hit, st, C.byref(cnt), nthreads)
+            return rgba, hit, cnt, st
         lib().orc_render(self.h, radii, colors4, pose, pane, W, H, pptr, npix, rgba, hit,
                          C.byref(cnt), nthreads)
         return rgba, hit, cnt

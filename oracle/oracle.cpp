@@ -421,6 +421,14 @@ float orc_pane_distance(int W, float fov_radians) {
 void orc_render(const void* tree, const float* radii, const float* colors, const float* pose,
                 float pane, int W, int H, const int32_t* pix, int64_t npix, float* rgba,
                 int32_t* hit, orc_counters* cnt, int nthreads) {
+    orc_render_steps(tree, radii, colors, pose, pane, W, H, pix, npix, rgba, hit, nullptr, cnt,
+                     nthreads);
+}
+
+void orc_render_steps(const void* tree, const float* radii, const float* colors,
+                      const float* pose, float pane, int W, int H, const int32_t* pix,
+                      int64_t npix, float* rgba, int32_t* hit, int32_t* steps,
+                      orc_counters* cnt, int nthreads) {
     const RefTree* t = static_cast<const RefTree*>(tree);
     int64_t total_pix = pix ? npix : (int64_t)W * H;
     orc_counters total{};
@@ -437,7 +445,9 @@ void orc_render(const void* tree, const float* radii, const float* colors, const
             float tt = ray.t_near;
             int32_t hid = -1;
             float depth = 0.0f;
+            int nsteps = 0;
             for (int i = 0; i < kMaxSteps; ++i) {
+                ++nsteps;
                 float px = ray.ox + ray.dx * tt;
                 float py = ray.oy + ray.dy * tt;
                 float pz = ray.oz + ray.dz * tt;
@@ -452,6 +462,7 @@ void orc_render(const void* tree, const float* radii, const float* colors, const
                 if (tt > ray.t_far) break;
             }
             hit[j] = hid;
+            if (steps) steps[j] = nsteps;
             if (hid >= 0) {
                 rgba[4 * j + 0] = colors[4 * (int64_t)hid + 0];
                 rgba[4 * j + 1] = colors[4 * (int64_t)hid + 1];

@@ -79,6 +79,11 @@ float orc_pane_distance(int W, float fov_radians);
 void orc_render(const void* tree, const float* radii, const float* colors, const float* pose,
                 float pane, int W, int H, const int32_t* pix, int64_t npix, float* rgba,
                 int32_t* hit, orc_counters* cnt, int nthreads);
+/* Same, also reporting the number of march steps of every pixel (steps: npix). */
+void orc_render_steps(const void* tree, const float* radii, const float* colors,
+                      const float* pose, float pane, int W, int H, const int32_t* pix,
+                      int64_t npix, float* rgba, int32_t* hit, int32_t* steps,
+                      orc_counters* cnt, int nthreads);
 
 /* Differentiable form, fp64 reference. With the hit map fixed, per hit pixel:
  *   depth = analytic ray/sphere entry distance of the hit sphere (centre c, radius r),

@@ -458,15 +458,17 @@ rm_tune default_tune() {
     t.warm = 1;
     t.waves_per_block = 2;
     t.blocks_per_cu = 0;
-    t.refill = 8;
+    t.refill = 1;
     return t;
 }
 
 bool tune_valid(const rm_tune& t) {
-    return t.tile_w > 0 && t.tile_h > 0 && t.tile_w * t.tile_h == 64 && (t.mode == 0 || t.mode == 1) &&
+    const int pix = t.tile_w * t.tile_h;
+    return t.tile_w > 0 && t.tile_h > 0 && pix >= 64 && pix <= 4096 && pix % 64 == 0 &&
+           (t.mode == 0 || t.mode == 1) &&
            (t.warm == 0 || t.warm == 1) &&
            (t.waves_per_block == 1 || t.waves_per_block == 2 || t.waves_per_block == 4) &&
-           t.blocks_per_cu >= 0 && t.blocks_per_cu <= 64 && t.refill >= 0 && t.refill <= 64;
+           t.blocks_per_cu >= 0 && t.blocks_per_cu <= 64 && t.refill >= 1 && t.refill <= 64;
 }
 
 // Lay out the views' tiles for a tune and upload the view table.
@@ -501,6 +503,7 @@ rm::BatchParams batch_params(rm_batch* b, int64_t sample_stride) {
     p.nviews = (int32_t)b->views.size();
     p.tile_w = b->tune.tile_w;
     p.tile_h = b->tune.tile_h;
+    p.refill_min = b->tune.refill;
     p.total_tiles = b->total_tiles;
     p.sample_stride = sample_stride;
     p.queue = b->d_queue;
@@ -514,7 +517,7 @@ int batch_enqueue(rm_batch* b, hipStream_t s, int64_t sample_stride) {
     rm_ctx* ctx = b->ctx;
     RM_HIP(ctx, rm::launch_reset_queue(s, b->d_queue));
     RM_HIP(ctx, rm::launch_batch(s, batch_params(b, sample_stride), b->tune.mode, b->tune.warm,
-                                 b->tune.waves_per_block, b->blocks, b->tune.refill));
+                                 b->tune.waves_per_block, b->blocks));
     return RM_OK;
 }
 
@@ -647,22 +650,17 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
             best = t;
         }
     };
-    // coordinate descent: traversal mode x warm start x schedule, then tile shape, then
-    // block shape, then the refill threshold
+    // coordinate descent: traversal mode x warm start, then tile shape and size (tiles above
+    // 64 pixels refill lanes locally), then block shape, then the refill threshold
     rm_tune base = default_tune();
-    // (refill is only tried with per-lane traversal: refilled lanes mix screen regions, which
-    // costs a packet traversal far more than it saves)
     for (int mode = 0; mode < 2; ++mode)
-        for (int warm = 0; warm < 2; ++warm)
-            for (int refill : {0, 8}) {
-                if (mode == 0 && refill > 0) continue;
-                rm_tune t = base;
-                t.mode = mode;
-                t.warm = warm;
-                t.refill = refill;
-                consider(t);
-            }
-    const int shapes[][2] = {{16, 4}, {4, 16}, {32, 2}};
+        for (int warm = 0; warm < 2; ++warm) {
+            rm_tune t = base;
+            t.mode = mode;
+            t.warm = warm;
+            consider(t);
+        }
+    const int shapes[][2] = {{8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 8}, {32, 16}, {32, 32}};
     rm_tune cur = best;
     for (auto& sh : shapes) {
         rm_tune t = cur;
@@ -671,7 +669,7 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         consider(t);
     }
     cur = best;
-    for (int wpb : {1, 4}) {
+    for (int wpb : {1, 2, 4}) {
         rm_tune t = cur;
         t.waves_per_block = wpb;
         consider(t);
@@ -683,8 +681,8 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         consider(t);
     }
     cur = best;
-    if (cur.refill > 0)
-        for (int rf : {1, 16, 32}) {
+    if (cur.tile_w * cur.tile_h > 64)
+        for (int rf : {4, 16, 32}) {
             rm_tune t = cur;
             t.refill = rf;
             consider(t);

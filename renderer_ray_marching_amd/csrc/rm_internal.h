@@ -53,7 +53,8 @@ struct BatchParams {
     DevCloud cloud;
     const DevView* views;
     int32_t nviews;
-    int32_t tile_w, tile_h;
+    int32_t tile_w, tile_h;   // tile_w * tile_h pixels per claimed tile (a multiple of 64)
+    int32_t refill_min;       // refill idle lanes from the tile once this many are idle
     int64_t total_tiles;
     int64_t sample_stride;  // process tiles 0, s, 2s, ... (1 = all)
     unsigned long long* queue;  // work counter (tiles, or pixels with refill)
@@ -75,7 +76,7 @@ hipError_t launch_set_colors(hipStream_t s, const float* colors, int64_t n, floa
 hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& v, MarchOut o,
                                int* err);
 hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
-                        int blocks, int refill);
+                        int blocks);
 hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q);
 hipError_t launch_refit(hipStream_t s, RmNode* nodes, float4* pts, float4* xyzr,
                         const float* xyz, int64_t n, const int32_t* level_order,

@@ -54,28 +54,77 @@ struct TopK {
         for (int j = 0; j < K; ++j) key[j] = pad;
     }
     __device__ __forceinline__ float worst_d2() const { return key_d2(key[K - 1]); }
-    __device__ __forceinline__ void insert(uint64_t kk) {
-        if (kk < key[K - 1]) {
-            key[K - 1] = kk;
+    // Insert kk < key[K-1]: with c[j] = kk < key[j] (monotone in j), slot j takes key[j-1]
+    // if kk goes before it, kk if kk goes exactly there, else keeps key[j]. K compares and
+    // 4 (K-1) + 2 selects, no data-dependent control flow.
+    __device__ __forceinline__ void insert_sorted(uint64_t kk) {
+        bool c[K];
 #pragma unroll
-            for (int j = K - 1; j > 0; --j) {
-                uint64_t a = key[j - 1], b = key[j];
-                bool sw = b < a;
-                key[j - 1] = sw ? b : a;
-                key[j] = sw ? a : b;
-            }
+        for (int j = 0; j < K; ++j) c[j] = kk < key[j];
+#pragma unroll
+        for (int j = K - 1; j > 0; --j) key[j] = c[j - 1] ? key[j - 1] : (c[j] ? kk : key[j]);
+        key[0] = c[0] ? kk : key[0];
+    }
+    __device__ __forceinline__ void insert(uint64_t kk) {
+        if (kk < key[K - 1]) insert_sorted(kk);
+    }
+    // Insert unless the same point is already listed (used when the list was seeded with real
+    // points: a seeded point met again in a leaf has the identical key).
+    __device__ __forceinline__ void insert_nodup(uint64_t kk) {
+        if (kk < key[K - 1]) {
+            const uint32_t id = key_idx(kk);
+            bool dup = false;
+#pragma unroll
+            for (int j = 0; j < K; ++j) dup |= key_idx(key[j]) == id;
+            if (!dup) insert_sorted(kk);
         }
+    }
+    template <bool NODUP>
+    __device__ __forceinline__ void add(uint64_t kk) {
+        if (NODUP)
+            insert_nodup(kk);
+        else
+            insert(kk);
     }
 };
 
+__device__ __forceinline__ void cmpx(uint64_t& a, uint64_t& b) {
+    const uint64_t lo = a < b ? a : b;
+    const uint64_t hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+// Sorts 10 keys: Batcher's odd-even merge sort network for 16 inputs restricted to the first 10
+// (the other six act as +infinity and never move); 32 compare-exchanges, checked exhaustively
+// with the 0-1 principle.
+__device__ __forceinline__ void sort10(uint64_t* k) {
+    cmpx(k[0], k[1]); cmpx(k[2], k[3]); cmpx(k[0], k[2]); cmpx(k[1], k[3]); cmpx(k[1], k[2]);
+    cmpx(k[4], k[5]); cmpx(k[6], k[7]); cmpx(k[4], k[6]); cmpx(k[5], k[7]); cmpx(k[5], k[6]);
+    cmpx(k[0], k[4]); cmpx(k[2], k[6]); cmpx(k[2], k[4]); cmpx(k[1], k[5]); cmpx(k[3], k[7]);
+    cmpx(k[3], k[5]); cmpx(k[1], k[2]); cmpx(k[3], k[4]); cmpx(k[5], k[6]); cmpx(k[8], k[9]);
+    cmpx(k[0], k[8]); cmpx(k[4], k[8]); cmpx(k[2], k[4]); cmpx(k[6], k[8]); cmpx(k[1], k[9]);
+    cmpx(k[5], k[9]); cmpx(k[3], k[5]); cmpx(k[7], k[9]); cmpx(k[1], k[2]); cmpx(k[3], k[4]);
+    cmpx(k[5], k[6]); cmpx(k[7], k[8]);
+}
+
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// ------------------------------------------------------------------------------------------
+// Stack entries (both traversals): an inner child is pushed as (node << 1), a leaf child as
+// (parent << 2 | slot << 1 | 1). A leaf's box is re-tested against the current k-th distance
+// when it is popped (the bound may have shrunk since the push, and a leaf scan is costly); an
+// inner child is expanded directly, since expanding it tests both of its children anyway.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int child_entry(int ni, int s, int ref, int cnt) {
+    return cnt > 0 ? ((ni << 2) | (s << 1) | 1) : (ref << 1);
+}
 
 // ------------------------------------------------------------------------------------------
 // Packet traversal: the whole wavefront walks one node sequence. A child is entered when any
 // active lane's mindist is within that lane's current k-th distance (ballot). Node records and
 // leaf points are read with scalar loads (the node index is wave-uniform) and broadcast to
-// every lane. The stack is wave-uniform and lives in LDS (one row of kWaveStack per wave);
-// an entry is (parent node << 1 | child slot) so the child box can be re-tested on pop.
+// every lane. The stack is wave-uniform and lives in LDS (one row of kWaveStack per wave).
 // ------------------------------------------------------------------------------------------
 template <int K>
 __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float qx, float qy,
@@ -83,30 +132,32 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
                                                      int* wstk, int& sp, int* err) {
     CFloat* f = nf + (int64_t)ni * 16;
     CInt* fi = (CInt*)f;
-    int c0 = fi[14], c1 = fi[15];
-    float w = tk.worst_d2();
-    float m0 = mindist(f[0], f[1], f[2], f[3], f[4], f[5], qx, qy, qz);
-    float m1 = mindist(f[6], f[7], f[8], f[9], f[10], f[11], qx, qy, qz);
-    uint64_t b0 = __ballot(active && c0 >= 0 && m0 <= w);
-    uint64_t b1 = __ballot(active && c1 >= 0 && m1 <= w);
+    const int r0 = fi[12], r1 = fi[13], c0 = fi[14], c1 = fi[15];
+    const float w = tk.worst_d2();
+    const float m0 = mindist(f[0], f[1], f[2], f[3], f[4], f[5], qx, qy, qz);
+    const float m1 = mindist(f[6], f[7], f[8], f[9], f[10], f[11], qx, qy, qz);
+    const uint64_t b0 = c0 >= 0 ? __builtin_amdgcn_ballot_w64(active & (m0 <= w)) : 0ull;
+    const uint64_t b1 = c1 >= 0 ? __builtin_amdgcn_ballot_w64(active & (m1 <= w)) : 0ull;
     if (sp + 2 > kWaveStack) {
         if (b0 | b1) *err = 1;
         return;
     }
+    const int e0 = child_entry(ni, 0, r0, c0), e1 = child_entry(ni, 1, r1, c1);
     if (b0 && b1) {
-        uint64_t pref0 = __ballot(active && m0 <= m1);
-        uint64_t act = __ballot(active);
-        int near = (2 * __popcll(pref0) >= __popcll(act)) ? 0 : 1;
-        wstk[sp++] = 2 * ni + (1 - near);
-        wstk[sp++] = 2 * ni + near;
+        // near child (popped first) = the one nearer for most active lanes
+        const uint64_t pref0 = __builtin_amdgcn_ballot_w64(active & (m0 <= m1));
+        const uint64_t act = __builtin_amdgcn_ballot_w64(active);
+        const bool near0 = 2 * __popcll(pref0) >= __popcll(act);
+        wstk[sp++] = near0 ? e1 : e0;
+        wstk[sp++] = near0 ? e0 : e1;
     } else if (b0) {
-        wstk[sp++] = 2 * ni;
+        wstk[sp++] = e0;
     } else if (b1) {
-        wstk[sp++] = 2 * ni + 1;
+        wstk[sp++] = e1;
     }
 }
 
-template <int K>
+template <int K, bool NODUP>
 __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, float qx, float qy,
                                                  float qz, TopK<K>& tk) {
     CFloat* p = pf + (int64_t)b * 4;
@@ -125,17 +176,18 @@ __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, flo
 #pragma unroll
         for (int u = 0; u < 4; ++u) d[u] = dist2(x[u], y[u], z[u], qx, qy, qz);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) tk.insert(make_key(d[u], id[u]));
+        for (int u = 0; u < 4; ++u) tk.template add<NODUP>(make_key(d[u], id[u]));
     }
     for (; j < cnt; ++j) {
         float d = dist2(p[4 * j], p[4 * j + 1], p[4 * j + 2], qx, qy, qz);
-        tk.insert(make_key(d, __float_as_uint(p[4 * j + 3])));
+        tk.template add<NODUP>(make_key(d, __float_as_uint(p[4 * j + 3])));
     }
 }
 
 // Exact top-K of every active lane's query. Must be called by the whole wavefront
-// (convergent); inactive lanes must hold keys that reject every insert (init(0)).
-template <int K>
+// (convergent); inactive lanes must hold keys that reject every insert (init(0)). NODUP: the
+// list was seeded with real points, so a candidate already listed is skipped.
+template <int K, bool NODUP = false>
 __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool active,
                            TopK<K>& tk, int* wstk, int* err) {
     CFloat* nf = (CFloat*)t.nodes;
@@ -144,18 +196,18 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
     packet_push_children<K>(nf, 0, qx, qy, qz, active, tk, wstk, sp, err);
     while (sp > 0) {
         --sp;
-        int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
-        int ni = e >> 1, s = e & 1;
-        CFloat* f = nf + (int64_t)ni * 16;
-        CInt* fi = (CInt*)f;
-        CFloat* bx = f + 6 * s;
-        float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
-        if (__ballot(active && md <= tk.worst_d2()) == 0) continue;
-        int ref = fi[12 + s], cnt = fi[14 + s];
-        if (cnt > 0)
-            packet_scan_leaf<K>(pf, ref, cnt, qx, qy, qz, tk);
-        else
-            packet_push_children<K>(nf, ref, qx, qy, qz, active, tk, wstk, sp, err);
+        const int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
+        if (e & 1) {
+            const int ni = e >> 2, s = (e >> 1) & 1;
+            CFloat* f = nf + (int64_t)ni * 16;
+            CInt* fi = (CInt*)f;
+            CFloat* bx = f + 6 * s;
+            const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
+            if (__builtin_amdgcn_ballot_w64(active & (md <= tk.worst_d2())) == 0) continue;
+            packet_scan_leaf<K, NODUP>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk);
+        } else {
+            packet_push_children<K>(nf, e >> 1, qx, qy, qz, active, tk, wstk, sp, err);
+        }
     }
 }
 
@@ -184,58 +236,60 @@ __device__ __forceinline__ void lane_push_children(const RmNode* nodes, int ni, 
                                                    int* lstk, int stride, int& sp, int maxsp,
                                                    int* err) {
     NodeRec n = load_node(nodes, ni);
-    float w = tk.worst_d2();
-    float m0 = mindist(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, qx, qy, qz);
-    float m1 = mindist(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, qx, qy, qz);
-    bool n0 = n.r.z >= 0 && m0 <= w;
-    bool n1 = n.r.w >= 0 && m1 <= w;
+    const float w = tk.worst_d2();
+    const float m0 = mindist(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, qx, qy, qz);
+    const float m1 = mindist(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, qx, qy, qz);
+    const bool n0 = (n.r.z >= 0) & (m0 <= w);
+    const bool n1 = (n.r.w >= 0) & (m1 <= w);
     if (sp + 2 > maxsp) {
         if (n0 || n1) *err = 1;
         return;
     }
+    const int e0 = child_entry(ni, 0, n.r.x, n.r.z), e1 = child_entry(ni, 1, n.r.y, n.r.w);
     if (n0 && n1) {
-        int near = m0 <= m1 ? 0 : 1;
-        lstk[(sp++) * stride] = 2 * ni + (1 - near);
-        lstk[(sp++) * stride] = 2 * ni + near;
+        const bool near0 = m0 <= m1;
+        lstk[(sp++) * stride] = near0 ? e1 : e0;
+        lstk[(sp++) * stride] = near0 ? e0 : e1;
     } else if (n0) {
-        lstk[(sp++) * stride] = 2 * ni;
+        lstk[(sp++) * stride] = e0;
     } else if (n1) {
-        lstk[(sp++) * stride] = 2 * ni + 1;
+        lstk[(sp++) * stride] = e1;
     }
 }
 
-template <int K>
+template <int K, bool NODUP = false>
 __device__ void knn_lane(const DevTree& t, float qx, float qy, float qz, TopK<K>& tk, int* lstk,
                          int stride, int maxsp, int* err) {
     int sp = 0;
     lane_push_children<K>(t.nodes, 0, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
     while (sp > 0) {
-        int e = lstk[(--sp) * stride];
-        int ni = e >> 1, s = e & 1;
+        const int e = lstk[(--sp) * stride];
+        if (!(e & 1)) {
+            lane_push_children<K>(t.nodes, e >> 1, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
+            continue;
+        }
+        const int ni = e >> 2, s = (e >> 1) & 1;
         NodeRec n = load_node(t.nodes, ni);
         float lx = s ? n.b.z : n.a.x, ly = s ? n.b.w : n.a.y, lz = s ? n.c.x : n.a.z;
         float hx = s ? n.c.y : n.a.w, hy = s ? n.c.z : n.b.x, hz = s ? n.c.w : n.b.y;
         int ref = s ? n.r.y : n.r.x, cnt = s ? n.r.w : n.r.z;
         float md = mindist(lx, ly, lz, hx, hy, hz, qx, qy, qz);
         if (md > tk.worst_d2()) continue;
-        if (cnt > 0) {
-            const float4* p = t.pts + ref;
-            int j = 0;
-            for (; j + 4 <= cnt; j += 4) {
-                float4 v[4];
+        const float4* p = t.pts + ref;
+        int j = 0;
+        for (; j + 4 <= cnt; j += 4) {
+            float4 v[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = p[j + u];
+            for (int u = 0; u < 4; ++u) v[u] = p[j + u];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    tk.insert(make_key(dist2(v[u].x, v[u].y, v[u].z, qx, qy, qz),
-                                       __float_as_uint(v[u].w)));
-            }
-            for (; j < cnt; ++j) {
-                float4 v = p[j];
-                tk.insert(make_key(dist2(v.x, v.y, v.z, qx, qy, qz), __float_as_uint(v.w)));
-            }
-        } else {
-            lane_push_children<K>(t.nodes, ref, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
+            for (int u = 0; u < 4; ++u)
+                tk.template add<NODUP>(make_key(dist2(v[u].x, v[u].y, v[u].z, qx, qy, qz),
+                                                __float_as_uint(v[u].w)));
+        }
+        for (; j < cnt; ++j) {
+            float4 v = p[j];
+            tk.template add<NODUP>(
+                make_key(dist2(v.x, v.y, v.z, qx, qy, qz), __float_as_uint(v.w)));
         }
     }
 }
@@ -379,20 +433,20 @@ __device__ __forceinline__ void sdf10(const DevCloud& c, const TopK<kMarchK>& tk
     i_out = bi;
 }
 
-// Warm start: the previous step's 10 neighbours, re-measured from the new query point, bound
-// the new 10th key from above (they are 10 valid candidates). Start the list with pads one
-// above that bound: only keys <= bound enter and pruning starts tight. Needs n >= 10.
-__device__ __forceinline__ void warm_init(const DevCloud& c, TopK<kMarchK>& tk, float qx,
+// Warm start: the previous step's 10 neighbours, re-measured from the new query point and
+// sorted, seed the list. They are 10 valid candidates, so the list's 10th key bounds the new 10th
+// key from above and pruning is tight from the first node; the traversal then only inserts
+// points not already listed (insert_nodup), and its result is the exact top 10. Needs n >= 10.
+__device__ __forceinline__ void warm_seed(const DevCloud& c, TopK<kMarchK>& tk, float qx,
                                           float qy, float qz) {
-    uint64_t bound = 0;
 #pragma unroll
     for (int j = 0; j < kMarchK; ++j) {
         uint32_t id = key_idx(tk.key[j]);
         float4 p = c.xyzr[id];
-        uint64_t kk = make_key(dist2(p.x, p.y, p.z, qx, qy, qz), id);
-        bound = kk > bound ? kk : bound;
+        tk.key[j] = make_key(dist2(p.x, p.y, p.z, qx, qy, qz), id);
     }
-    tk.init(bound + 1);
+    static_assert(kMarchK == 10, "sort10 sorts ten keys");
+    sort10(tk.key);
 }
 
 template <bool WARM>
@@ -407,11 +461,13 @@ __device__ void march_lane(const DevCloud& c, const Ray& r, int* lstk, int strid
         float px = r.ox + r.dx * t;
         float py = r.oy + r.dy * t;
         float pz = r.oz + r.dz * t;
-        if (WARM && i > 0)
-            warm_init(c, tk, px, py, pz);
-        else
+        if (WARM && i > 0) {
+            warm_seed(c, tk, px, py, pz);
+            knn_lane<kMarchK, true>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
+        } else {
             tk.init(kPadKey);
-        knn_lane<kMarchK>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
+            knn_lane<kMarchK>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
+        }
         float s;
         uint32_t id;
         sdf10(c, tk, s, id);
@@ -422,44 +478,6 @@ __device__ void march_lane(const DevCloud& c, const Ray& r, int* lstk, int strid
             return;
         }
         if (t > r.tfar) return;
-    }
-}
-
-template <bool WARM>
-__device__ void march_packet(const DevCloud& c, const Ray& r, bool valid, int* wstk, int* err,
-                             int& hit, float& depth) {
-    TopK<kMarchK> tk;
-    tk.init(kPadKey);
-    bool active = valid;
-    float t = r.tnear;
-    int steps = 0;
-    hit = -1;
-    depth = 0.0f;
-    while (__ballot(active)) {
-        float px = r.ox + r.dx * t;
-        float py = r.oy + r.dy * t;
-        float pz = r.oz + r.dz * t;
-        if (!active)
-            tk.init(0ull);
-        else if (WARM && steps > 0)
-            warm_init(c, tk, px, py, pz);
-        else
-            tk.init(kPadKey);
-        knn_packet<kMarchK>(c.tree, px, py, pz, active, tk, wstk, err);
-        if (active) {
-            float s;
-            uint32_t id;
-            sdf10(c, tk, s, id);
-            t = t + s;
-            ++steps;
-            if (s < kMinDist) {
-                hit = (int)id;
-                depth = t;
-                active = false;
-            } else if (t > r.tfar || steps >= kMaxSteps) {
-                active = false;
-            }
-        }
     }
 }
 
@@ -490,24 +508,38 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
     write_pixel(c, o, v.pix_offset + (int64_t)y * v.W + x, hit, depth);
 }
 
-// Batched path: a persistent grid; each wavefront takes 64-pixel tiles from one shared
-// queue over all views until the queue is empty, and marches the tile as one packet.
+// Batched path: a persistent grid over all views. A wavefront claims one screen tile
+// (tile_w x tile_h pixels, a multiple of 64) from a shared 64-bit queue and keeps 64 rays of that
+// tile in flight: a lane whose ray ends takes the tile's next pixel (claimed for every idle lane
+// at once by ballot + prefix count, once at least `refill_min` lanes are idle), so rays in a
+// wavefront stay screen neighbours and the tail of long rays is paid once per tile, not once per
+// 64 pixels. One wavefront step advances every live ray by one march step (one kNN-10 query
+// each), in packet (MODE 0) or per-lane (MODE 1) traversal.
+// For autotuning, `sample_stride` > 1 keeps chunks of 16384 consecutive tiles out of every
+// 16384 * sample_stride: a chunk is larger than the set of tiles in flight at once, so the
+// sample has the cache locality of the full launch.
 template <int MODE, bool WARM, int WPB>
 __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
     extern __shared__ int lstk[];
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     CFloat* vf = (CFloat*)p.views;
     constexpr int VSTRIDE = sizeof(DevView) / 4;
+    constexpr int64_t kChunk = 16384;
+    const int tile_w = p.tile_w, tile_pix = p.tile_w * p.tile_h;
     while (true) {
         unsigned long long qv = 0;
         if (lane == 0) qv = atomicAdd(p.queue, 1ull);
         qv = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(qv >> 32)) << 32) |
              (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qv);
-        int64_t tile = (int64_t)qv * p.sample_stride;
+        const int64_t q = (int64_t)qv;
+        const int64_t tile = p.sample_stride > 1
+                                 ? (q / kChunk) * kChunk * p.sample_stride + q % kChunk
+                                 : q;
         if (tile >= p.total_tiles) break;
-        // view lookup: last view whose tile_offset <= tile
+        // view lookup: last view whose tile_offset <= tile (scalar loads)
         int lo = 0, hi = p.nviews - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;
@@ -526,115 +558,74 @@ __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxs
             *(const __attribute__((address_space(4))) int64_t*)(V + offsetof(DevView, pix_offset) / 4);
         const int64_t tile_off =
             *(const __attribute__((address_space(4))) int64_t*)(V + offsetof(DevView, tile_offset) / 4);
-        int64_t lt = tile - tile_off;
-        int tx = (int)(lt % tiles_x), ty = (int)(lt / tiles_x);
-        int x = tx * p.tile_w + lane % p.tile_w;
-        int y = ty * p.tile_h + lane / p.tile_w;
-        bool valid = x < W && y < H;
-        Ray r = make_ray([&](int i) { return V[i]; }, pane, W, H, valid ? x : 0, valid ? y : 0);
-        int hit = -1;
-        float depth = 0.0f;
-        if (MODE == 0) {
-            march_packet<WARM>(p.cloud, r, valid, wstk[w], p.err, hit, depth);
-        } else {
-            if (valid)
-                march_lane<WARM>(p.cloud, r, &lstk[threadIdx.x], WPB * 64, maxsp, p.err, hit,
-                                 depth);
-        }
-        if (valid) write_pixel(p.cloud, p.out, pix_off + (int64_t)y * W + x, hit, depth);
-    }
-}
-
-// Batched path with per-lane refill: a lane whose ray has ended takes the next pixel of the
-// shared queue (pixels numbered tile by tile, so lanes refilled together stay close on screen),
-// and the wavefront keeps stepping until the queue is drained. One queue claim serves every
-// lane that needs work (ballot + prefix count), and claims wait until at least `refill_min`
-// lanes are idle so the counter sees few atomics. A wavefront step advances every live ray by
-// one march step (one kNN-10 query each), in packet (MODE 0) or per-lane (MODE 1) traversal.
-template <int MODE, bool WARM, int WPB>
-__global__ __launch_bounds__(WPB * 64) void batch_refill_kernel(BatchParams p, int maxsp,
-                                                                int refill_min) {
-    __shared__ int wstk[WPB][kWaveStack];
-    extern __shared__ int lstk[];
-    const int w = threadIdx.x >> 6;
-    const int lane = lane_id();
-    const int64_t ntiles = (p.total_tiles + p.sample_stride - 1) / p.sample_stride;
-    const int64_t nwork = ntiles * 64;
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    bool has = false;
-    bool drained = false;          // wave-uniform
-    Ray r = {0.f, 0.f, 0.f, 0.f, 0.f, -1.f, 0.f, 0.f};
-    float t = 0.0f;
-    int steps = 0;
-    int64_t out = 0;
-    TopK<kMarchK> tk;
-    tk.init(kPadKey);
-    while (true) {
-        uint64_t needm = __ballot(!has);
-        int nneed = __popcll(needm);
-        if (!drained && (nneed >= refill_min || nneed == 64)) {
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(p.queue, (unsigned long long)nneed);
-            base = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32))
-                    << 32) | (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)base);
-            if ((int64_t)base + nneed >= nwork) drained = true;
-            if (!has) {
-                int64_t wi = (int64_t)base + __popcll(needm & lt_mask);
-                if (wi < nwork) {
-                    int64_t tile = (wi >> 6) * p.sample_stride;
-                    int sub = (int)(wi & 63);
-                    int lo = 0, hi = p.nviews - 1;
-                    while (lo < hi) {
-                        int mid = (lo + hi + 1) >> 1;
-                        if (p.views[mid].tile_offset <= tile)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
-                    }
-                    const DevView* V = p.views + lo;
-                    const int W = V->W, H = V->H, tiles_x = V->tiles_x;
-                    int64_t lt = tile - V->tile_offset;
-                    int x = (int)(lt % tiles_x) * p.tile_w + sub % p.tile_w;
-                    int y = (int)(lt / tiles_x) * p.tile_h + sub / p.tile_w;
+        const int64_t lt = tile - tile_off;
+        const int x0 = (int)(lt % tiles_x) * tile_w;
+        const int y0 = (int)(lt / tiles_x) * p.tile_h;
+        int next = 0;                      // wave-uniform: next unclaimed pixel of the tile
+        bool has = false;
+        Ray r = {0.f, 0.f, 0.f, 0.f, 0.f, -1.f, 0.f, 0.f};
+        float t = 0.0f;
+        int steps = 0;
+        int64_t out = 0;
+        TopK<kMarchK> tk;
+        tk.init(kPadKey);
+        while (true) {
+            const uint64_t needm = __builtin_amdgcn_ballot_w64(!has);
+            const int nneed = __popcll(needm);
+            if (next < tile_pix && (nneed >= p.refill_min || nneed == 64)) {
+                const int my = next + __popcll(needm & lt_mask);
+                if (!has && my < tile_pix) {
+                    const int x = x0 + my % tile_w, y = y0 + my / tile_w;
                     if (x < W && y < H) {
-                        r = make_ray([&](int i) { return V->M[i]; }, V->pane, W, H, x, y);
+                        r = make_ray([&](int i) { return V[i]; }, pane, W, H, x, y);
                         t = r.tnear;
                         steps = 0;
-                        out = V->pix_offset + (int64_t)y * W + x;
+                        out = pix_off + (int64_t)y * W + x;
                         has = true;
                     }
                 }
+                next += nneed;
             }
-        }
-        if (__ballot(has) == 0) {
-            if (drained) break;
-            continue;
-        }
-        float px = r.ox + r.dx * t;
-        float py = r.oy + r.dy * t;
-        float pz = r.oz + r.dz * t;
-        if (!has)
-            tk.init(0ull);
-        else if (WARM && steps > 0)
-            warm_init(p.cloud, tk, px, py, pz);
-        else
-            tk.init(kPadKey);
-        if (MODE == 0) {
-            knn_packet<kMarchK>(p.cloud.tree, px, py, pz, has, tk, wstk[w], p.err);
-        } else if (has) {
-            knn_lane<kMarchK>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x], WPB * 64, maxsp,
-                              p.err);
-        }
-        if (has) {
-            float s;
-            uint32_t id;
-            sdf10(p.cloud, tk, s, id);
-            t = t + s;
-            ++steps;
-            bool hitnow = s < kMinDist;
-            if (hitnow || t > r.tfar || steps >= kMaxSteps) {
-                write_pixel(p.cloud, p.out, out, hitnow ? (int)id : -1, hitnow ? t : 0.0f);
-                has = false;
+            if (__builtin_amdgcn_ballot_w64(has) == 0) {
+                if (next >= tile_pix) break;
+                continue;
+            }
+            const float px = r.ox + r.dx * t;
+            const float py = r.oy + r.dy * t;
+            const float pz = r.oz + r.dz * t;
+            const bool warm_lane = WARM && has && steps > 0;
+            if (!has)
+                tk.init(0ull);
+            else if (warm_lane)
+                warm_seed(p.cloud, tk, px, py, pz);
+            else
+                tk.init(kPadKey);
+            if (MODE == 0) {
+                // a fresh lane's pads (index 0xffffffff) never match a real candidate, so the
+                // duplicate check is harmless for it and one traversal serves both kinds
+                if (WARM && __builtin_amdgcn_ballot_w64(warm_lane) != 0)
+                    knn_packet<kMarchK, true>(p.cloud.tree, px, py, pz, has, tk, wstk[w], p.err);
+                else
+                    knn_packet<kMarchK>(p.cloud.tree, px, py, pz, has, tk, wstk[w], p.err);
+            } else if (has) {
+                if (warm_lane)
+                    knn_lane<kMarchK, true>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
+                                            WPB * 64, maxsp, p.err);
+                else
+                    knn_lane<kMarchK>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
+                                      WPB * 64, maxsp, p.err);
+            }
+            if (has) {
+                float s;
+                uint32_t id;
+                sdf10(p.cloud, tk, s, id);
+                t = t + s;
+                ++steps;
+                const bool hitnow = s < kMinDist;
+                if (hitnow || t > r.tfar || steps >= kMaxSteps) {
+                    write_pixel(p.cloud, p.out, out, hitnow ? (int)id : -1, hitnow ? t : 0.0f);
+                    has = false;
+                }
             }
         }
     }
@@ -917,23 +908,18 @@ hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& 
 }
 
 template <int MODE, bool WARM, int WPB>
-static hipError_t launch_batch_t(hipStream_t s, const BatchParams& p, int blocks, int refill) {
+static hipError_t launch_batch_t(hipStream_t s, const BatchParams& p, int blocks) {
     int maxsp = lane_stack_depth(p.cloud.tree);
     size_t shmem = MODE == 1 ? (size_t)maxsp * WPB * 64 * sizeof(int) : 0;
-    if (refill > 0)
-        hipLaunchKernelGGL((batch_refill_kernel<MODE, WARM, WPB>), dim3(blocks), dim3(WPB * 64),
-                           shmem, s, p, maxsp, refill);
-    else
-        hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB>), dim3(blocks), dim3(WPB * 64), shmem,
-                           s, p, maxsp);
+    hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB>), dim3(blocks), dim3(WPB * 64), shmem, s,
+                       p, maxsp);
     return hipGetLastError();
 }
 
 hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
-                        int blocks, int refill) {
+                        int blocks) {
 #define RM_B(M, W, P) \
-    if (mode == M && (warm != 0) == W && wpb == P) \
-        return launch_batch_t<M, W, P>(s, p, blocks, refill);
+    if (mode == M && (warm != 0) == W && wpb == P) return launch_batch_t<M, W, P>(s, p, blocks);
     RM_B(0, false, 1) RM_B(0, false, 2) RM_B(0, false, 4)
     RM_B(0, true, 1) RM_B(0, true, 2) RM_B(0, true, 4)
     RM_B(1, false, 1) RM_B(1, false, 2) RM_B(1, false, 4)

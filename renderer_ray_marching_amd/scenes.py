@@ -293,6 +293,15 @@ def workload_room_2m(n_views: int = 32):
     return cloud, room_views(n_views, spec, seed=102, sizes=((1067, 690, 803.8),)), 0.05
 
 
+def workload_box_1m_diff(n_views: int = 16):
+    """Differentiable workload: 1M points on the box scene (seed 4), 16 views at 512x512,
+    fx = fy = 450, each 3 m from the centroid at its own azimuth."""
+    cloud = box_scene(1_000_000, seed=4)
+    views = [box_scene_view(cloud, 512, 512, 450.0, distance=3.0, seed=400 + i)
+             for i in range(n_views)]
+    return cloud, views, 0.1
+
+
 def room_10m_spec() -> RoomSpec:
     return RoomSpec(size=(20.0, 16.0, 3.0), n_boxes=800)
 

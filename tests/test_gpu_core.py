@@ -242,15 +242,16 @@ def test_plain_march_tiny_cloud_and_misses(rm_ctx):
 # ------------------------------------------------------------------------------------------
 
 TUNES = [
-    dict(tile_w=8, tile_h=8, mode=0, warm=0, waves_per_block=2, blocks_per_cu=0, refill=0),
-    dict(tile_w=8, tile_h=8, mode=0, warm=1, waves_per_block=1, blocks_per_cu=0, refill=0),
-    dict(tile_w=16, tile_h=4, mode=0, warm=1, waves_per_block=4, blocks_per_cu=2, refill=0),
-    dict(tile_w=32, tile_h=2, mode=1, warm=0, waves_per_block=2, blocks_per_cu=0, refill=0),
-    dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=4, blocks_per_cu=1, refill=0),
-    dict(tile_w=8, tile_h=8, mode=0, warm=1, waves_per_block=2, blocks_per_cu=0, refill=8),
-    dict(tile_w=4, tile_h=16, mode=0, warm=0, waves_per_block=1, blocks_per_cu=3, refill=1),
-    dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=2, blocks_per_cu=0, refill=8),
-    dict(tile_w=16, tile_h=4, mode=1, warm=0, waves_per_block=4, blocks_per_cu=16, refill=64),
+    dict(tile_w=8, tile_h=8, mode=0, warm=0, waves_per_block=2, blocks_per_cu=0, refill=1),
+    dict(tile_w=8, tile_h=8, mode=0, warm=1, waves_per_block=1, blocks_per_cu=0, refill=1),
+    dict(tile_w=16, tile_h=4, mode=0, warm=1, waves_per_block=4, blocks_per_cu=2, refill=1),
+    dict(tile_w=32, tile_h=2, mode=1, warm=0, waves_per_block=2, blocks_per_cu=0, refill=1),
+    dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=4, blocks_per_cu=1, refill=1),
+    dict(tile_w=16, tile_h=16, mode=0, warm=1, waves_per_block=2, blocks_per_cu=0, refill=1),
+    dict(tile_w=32, tile_h=8, mode=0, warm=0, waves_per_block=1, blocks_per_cu=3, refill=16),
+    dict(tile_w=16, tile_h=8, mode=1, warm=1, waves_per_block=2, blocks_per_cu=0, refill=4),
+    dict(tile_w=64, tile_h=64, mode=1, warm=0, waves_per_block=4, blocks_per_cu=16, refill=64),
+    dict(tile_w=24, tile_h=8, mode=0, warm=1, waves_per_block=2, blocks_per_cu=0, refill=8),
 ]
 
 
