@@ -141,7 +141,8 @@ typedef struct rm_tune {
  * views' offsets and sizes; they are bound at creation (the graph captures them). */
 int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
                     const rm_tune* tune, float* rgba, int32_t* hit, rm_batch** out);
-/* Time candidate rm_tune settings on a 1/sample_stride subset of the tiles and keep the
+/* Time candidate rm_tune settings on every sample_stride-th view (whole views, so the sample
+ * keeps each view's screen locality and the mix of view sizes) and keep the
  * fastest (re-captures the graph). chosen may be NULL. */
 int rm_batch_autotune(rm_batch* batch, int sample_stride, rm_tune* chosen);
 int rm_batch_get_tune(const rm_batch* batch, rm_tune* tune);

@@ -57,7 +57,7 @@ struct rm_batch {
     std::vector<rm_view> views;
     DevView* d_views = nullptr;
     int64_t total_tiles = 0;
-    int64_t total_pixels = 0;
+    int laid_views = 0;
     rm_tune tune{};
     unsigned long long* d_queue = nullptr;
     float* rgba = nullptr;
@@ -246,12 +246,12 @@ float rm_ctx_last_kernel_ms(const rm_ctx* ctx) {
 // ---------------------------------------------------------------------------------------------
 // tree
 // ---------------------------------------------------------------------------------------------
-int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tree** out) {
-    if (!ctx || !out) return RM_ERR_INVALID;
-    *out = nullptr;
-    if (!xyzw || n <= 0) return fail(ctx, RM_ERR_INVALID, "rm_tree_build: empty input");
-    if (leaf_max == 0) leaf_max = kDefaultLeaf;
-    if (int r = set_device(ctx)) return r;
+}  // extern "C"
+
+namespace {
+
+// Host build (rm_tree_host.cpp): kept for cross-checks and as a fallback (RM_TREE_BUILD=host).
+int tree_build_host(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tree* t) {
     std::vector<float> host((size_t)n * 4);
     RM_HIP(ctx, hipMemcpyAsync(host.data(), xyzw, (size_t)n * 16, hipMemcpyDefault, ctx->stream));
     RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -260,14 +260,8 @@ int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tr
     if (!rm_build_host_tree(host.data(), n, leaf_max, ht, msg)) return fail(ctx, RM_ERR_INVALID, msg);
     if (ht.depth > rm::kMaxDepth)
         return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: tree deeper than supported");
-    rm_tree* t = new (std::nothrow) rm_tree();
-    if (!t) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");
-    t->device = ctx->device;
-    t->n = n;
     t->n_nodes = (int64_t)ht.nodes.size();
     t->depth = ht.depth;
-    t->leaf_max = leaf_max;
-    // level grouping for refits
     int nlev = 0;
     for (int32_t l : ht.level) nlev = std::max(nlev, l + 1);
     t->level_start.assign(nlev + 1, 0);
@@ -283,21 +277,119 @@ int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tr
         hipMalloc(&t->level_order, sizeof(int32_t) * std::max<size_t>(1, order.size())) !=
             hipSuccess) {
         (void)hipGetLastError();
-        rm_tree_destroy(t);
         return fail(ctx, RM_ERR_NOMEM, "rm_tree_build: device allocation failed");
     }
-    hipError_t e = hipMemcpyAsync(t->nodes, ht.nodes.data(), sizeof(RmNode) * ht.nodes.size(),
-                                  hipMemcpyHostToDevice, ctx->stream);
+    RM_HIP(ctx, hipMemcpyAsync(t->nodes, ht.nodes.data(), sizeof(RmNode) * ht.nodes.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    RM_HIP(ctx, hipMemcpyAsync(t->pts, ht.pts.data(), sizeof(float4) * (size_t)n,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!order.empty())
+        RM_HIP(ctx, hipMemcpyAsync(t->level_order, order.data(), sizeof(int32_t) * order.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RM_OK;
+}
+
+// Device build: the host lays out the shape (counts only), the GPU sorts, partitions and fills
+// in boxes level by level (rm_kernels.hip, build_tree_device).
+int tree_build_device(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tree* t) {
+    RmShape shape;
+    std::string msg;
+    if (!rm_tree_shape(n, leaf_max, shape, msg)) return fail(ctx, RM_ERR_INVALID, msg);
+    if (shape.depth > rm::kMaxDepth)
+        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: tree deeper than supported");
+    const size_t nn = shape.nodes.size();
+    t->n_nodes = (int64_t)nn;
+    t->depth = shape.depth;
+    t->level_start = shape.level_start;
+    std::vector<int32_t> order(nn);
+    for (size_t i = 0; i < nn; ++i) order[i] = (int32_t)i;  // breadth-first = level order
+    if (hipMalloc(&t->nodes, sizeof(RmNode) * nn) != hipSuccess ||
+        hipMalloc(&t->pts, sizeof(float4) * (size_t)n) != hipSuccess ||
+        hipMalloc(&t->level_order, sizeof(int32_t) * nn) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, RM_ERR_NOMEM, "rm_tree_build: device allocation failed");
+    }
+    const size_t temp = rm::build_temp_bytes(n);
+    const size_t n4 = align_up(sizeof(int32_t) * (size_t)n);
+    const size_t total = align_up(sizeof(float4) * (size_t)n) + 15 * n4 + align_up(nn) +
+                         3 * align_up(sizeof(int64_t) * nn) + align_up(temp);
+    char* base = nullptr;
+    if (hipMalloc(&base, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, RM_ERR_NOMEM, "rm_tree_build: scratch allocation failed");
+    }
+    char* c = base;
+    auto take = [&](size_t bytes) {
+        char* r = c;
+        c += align_up(bytes);
+        return r;
+    };
+    float* in = reinterpret_cast<float*>(take(sizeof(float4) * (size_t)n));
+    rm::BuildScratch w;
+    w.keys = reinterpret_cast<uint32_t*>(take(n4));
+    w.keys_out = reinterpret_cast<uint32_t*>(take(n4));
+    w.ids = reinterpret_cast<int32_t*>(take(n4));
+    for (int d = 0; d < 3; ++d)
+        for (int b = 0; b < 2; ++b) w.list[d][b] = reinterpret_cast<int32_t*>(take(n4));
+    w.flag = reinterpret_cast<int32_t*>(take(n4));
+    w.il = reinterpret_cast<int32_t*>(take(n4));
+    w.pl = reinterpret_cast<int32_t*>(take(n4));
+    w.owner = reinterpret_cast<int32_t*>(take(n4));
+    w.axis = reinterpret_cast<int8_t*>(take(nn));
+    int64_t* seg_b = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * nn));
+    int64_t* seg_e = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * nn));
+    int32_t* nl = reinterpret_cast<int32_t*>(take(sizeof(int64_t) * nn));
+    w.temp = take(temp);
+    w.temp_bytes = temp;
+    hipError_t e = hipMemcpyAsync(in, xyzw, sizeof(float4) * (size_t)n, hipMemcpyDefault,
+                                  ctx->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(t->pts, ht.pts.data(), sizeof(float4) * (size_t)n,
+        e = hipMemcpyAsync(t->nodes, shape.nodes.data(), sizeof(RmNode) * nn,
                            hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess && !order.empty())
-        e = hipMemcpyAsync(t->level_order, order.data(), sizeof(int32_t) * order.size(),
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->level_order, order.data(), sizeof(int32_t) * nn,
                            hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(seg_b, shape.seg_b.data(), sizeof(int64_t) * nn,
+                           hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(seg_e, shape.seg_e.data(), sizeof(int64_t) * nn,
+                           hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(nl, shape.nl.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess)
+        e = rm::build_tree_device(ctx->stream, in, shape, seg_b, seg_e, nl, t->nodes, t->pts, w);
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(base);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rm_tree_build (device)");
+    if (e2 != hipSuccess) return hip_fail(ctx, e2, "rm_tree_build (device)");
+    return RM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tree** out) {
+    if (!ctx || !out) return RM_ERR_INVALID;
+    *out = nullptr;
+    if (!xyzw || n <= 0) return fail(ctx, RM_ERR_INVALID, "rm_tree_build: empty input");
+    if (leaf_max == 0) leaf_max = kDefaultLeaf;
+    if (int r = set_device(ctx)) return r;
+    rm_tree* t = new (std::nothrow) rm_tree();
+    if (!t) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");
+    t->device = ctx->device;
+    t->n = n;
+    t->leaf_max = leaf_max;
+    const char* env = std::getenv("RM_TREE_BUILD");
+    const bool host = env && std::strcmp(env, "host") == 0;
+    int r = host ? tree_build_host(ctx, xyzw, n, leaf_max, t)
+                 : tree_build_device(ctx, xyzw, n, leaf_max, t);
+    if (r != RM_OK) {
         rm_tree_destroy(t);
-        return hip_fail(ctx, e, "rm_tree_build upload");
+        return r;
     }
     *out = t;
     return RM_OK;
@@ -472,17 +564,21 @@ bool tune_valid(const rm_tune& t) {
 }
 
 // Lay out the views' tiles for a tune and upload the view table.
-int batch_layout(rm_batch* b) {
+// Lays out the tiles of every `view_stride`-th view (1 = all; the autotuner times a subset of
+// whole views, which keeps each view's screen locality and the mix of view sizes).
+int batch_layout(rm_batch* b, int view_stride = 1) {
     rm_ctx* ctx = b->ctx;
-    std::vector<DevView> dv(b->views.size());
+    std::vector<DevView> dv;
     int64_t tiles = 0;
-    for (size_t i = 0; i < b->views.size(); ++i) {
+    for (size_t i = 0; i < b->views.size(); i += (size_t)view_stride) {
         const rm_view& v = b->views[i];
-        dv[i] = dev_view(v.pose, v.fov_radians, v.width, v.height, v.offset, b->tune.tile_w, tiles);
+        dv.push_back(dev_view(v.pose, v.fov_radians, v.width, v.height, v.offset,
+                              b->tune.tile_w, tiles));
         int64_t ty = (v.height + b->tune.tile_h - 1) / b->tune.tile_h;
-        tiles += (int64_t)dv[i].tiles_x * ty;
+        tiles += (int64_t)dv.back().tiles_x * ty;
     }
     b->total_tiles = tiles;
+    b->laid_views = (int)dv.size();
     if (tiles >= (int64_t)UINT32_MAX) return fail(ctx, RM_ERR_UNSUPPORTED, "too many tiles");
     RM_HIP(ctx, hipMemcpyAsync(b->d_views, dv.data(), sizeof(DevView) * dv.size(),
                                hipMemcpyHostToDevice, ctx->stream));
@@ -496,16 +592,15 @@ int batch_layout(rm_batch* b) {
     return RM_OK;
 }
 
-rm::BatchParams batch_params(rm_batch* b, int64_t sample_stride) {
+rm::BatchParams batch_params(rm_batch* b) {
     rm::BatchParams p;
     p.cloud = dev_cloud(b->cloud);
     p.views = b->d_views;
-    p.nviews = (int32_t)b->views.size();
+    p.nviews = (int32_t)b->laid_views;
     p.tile_w = b->tune.tile_w;
     p.tile_h = b->tune.tile_h;
     p.refill_min = b->tune.refill;
     p.total_tiles = b->total_tiles;
-    p.sample_stride = sample_stride;
     p.queue = b->d_queue;
     p.err = b->ctx->d_err;
     p.out.rgba = reinterpret_cast<float4*>(b->rgba);
@@ -513,10 +608,10 @@ rm::BatchParams batch_params(rm_batch* b, int64_t sample_stride) {
     return p;
 }
 
-int batch_enqueue(rm_batch* b, hipStream_t s, int64_t sample_stride) {
+int batch_enqueue(rm_batch* b, hipStream_t s) {
     rm_ctx* ctx = b->ctx;
     RM_HIP(ctx, rm::launch_reset_queue(s, b->d_queue));
-    RM_HIP(ctx, rm::launch_batch(s, batch_params(b, sample_stride), b->tune.mode, b->tune.warm,
+    RM_HIP(ctx, rm::launch_batch(s, batch_params(b), b->tune.mode, b->tune.warm,
                                  b->tune.waves_per_block, b->blocks));
     return RM_OK;
 }
@@ -534,7 +629,7 @@ int batch_capture(rm_batch* b) {
     // Capture on the context's own stream (the null stream cannot be captured); the graph is
     // launched on whatever stream the context works on.
     RM_HIP(ctx, hipStreamBeginCapture(ctx->own, hipStreamCaptureModeThreadLocal));
-    int r = batch_enqueue(b, ctx->own, 1);
+    int r = batch_enqueue(b, ctx->own);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->own, &g);
     if (r != RM_OK) {
@@ -615,11 +710,11 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
     RM_HIP(ctx, hipEventCreate(&e1));
     auto measure = [&](const rm_tune& t, float& ms) -> int {
         b->tune = t;
-        if (int r = batch_layout(b)) return r;
+        if (int r = batch_layout(b, sample_stride)) return r;
         float best = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
             RM_HIP(ctx, hipEventRecord(e0, ctx->stream));
-            if (int r = batch_enqueue(b, ctx->stream, sample_stride)) return r;
+            if (int r = batch_enqueue(b, ctx->stream)) return r;
             RM_HIP(ctx, hipEventRecord(e1, ctx->stream));
             RM_HIP(ctx, hipEventSynchronize(e1));
             float v = 0.0f;
@@ -660,7 +755,7 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
             t.warm = warm;
             consider(t);
         }
-    const int shapes[][2] = {{8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 8}, {32, 16}, {32, 32}};
+    const int shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {16, 8}};
     rm_tune cur = best;
     for (auto& sh : shapes) {
         rm_tune t = cur;
@@ -675,7 +770,7 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         consider(t);
     }
     cur = best;
-    for (int bpc : {2, 4, 16}) {
+    for (int bpc : {4, 16}) {
         rm_tune t = cur;
         t.blocks_per_cu = bpc;
         consider(t);

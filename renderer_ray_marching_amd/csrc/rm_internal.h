@@ -56,7 +56,6 @@ struct BatchParams {
     int32_t tile_w, tile_h;   // tile_w * tile_h pixels per claimed tile (a multiple of 64)
     int32_t refill_min;       // refill idle lanes from the tile once this many are idle
     int64_t total_tiles;
-    int64_t sample_stride;  // process tiles 0, s, 2s, ... (1 = all)
     unsigned long long* queue;  // work counter (tiles, or pixels with refill)
     int* err;               // device error flag
     MarchOut out;
@@ -90,5 +89,26 @@ hipError_t launch_adjoint_reduce(hipStream_t s, const DevCloud& c, const DevView
                                  const int32_t* keys_sorted, const int32_t* vals_sorted,
                                  int64_t npix, const float* grad_rgb, const float* grad_depth,
                                  float* g_xyz, float* g_r, float* g_rgb);
+
+// Scratch of the device tree build (all n-sized unless noted).
+struct BuildScratch {
+    uint32_t* keys;
+    uint32_t* keys_out;
+    int32_t* ids;
+    int32_t* list[3][2];    // per axis, double-buffered
+    int32_t* flag;
+    int32_t* il;
+    int32_t* pl;
+    int32_t* owner;
+    int8_t* axis;           // per inner node
+    void* temp;             // hipcub temp storage, temp_bytes
+    size_t temp_bytes;
+};
+
+size_t build_temp_bytes(int64_t n);
+hipError_t build_tree_device(hipStream_t s, const float* xyzw, const RmShape& shape,
+                             const int64_t* d_seg_b, const int64_t* d_seg_e,
+                             const int32_t* d_nl, RmNode* d_nodes, float4* d_pts,
+                             BuildScratch& w);
 
 }  // namespace rm

@@ -9,6 +9,8 @@
 //   pruning           a child box is visited iff its mindist <= the current k-th d2; mindist
 //                     is a float lower bound of every candidate d2 inside the box, so the
 //                     result is the exact top-k under the total order, whatever the tree.
+#include <hipcub/hipcub.hpp>
+
 #include "rm_internal.h"
 
 namespace rm {
@@ -515,9 +517,6 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
 // wavefront stay screen neighbours and the tail of long rays is paid once per tile, not once per
 // 64 pixels. One wavefront step advances every live ray by one march step (one kNN-10 query
 // each), in packet (MODE 0) or per-lane (MODE 1) traversal.
-// For autotuning, `sample_stride` > 1 keeps chunks of 16384 consecutive tiles out of every
-// 16384 * sample_stride: a chunk is larger than the set of tiles in flight at once, so the
-// sample has the cache locality of the full launch.
 template <int MODE, bool WARM, int WPB>
 __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
@@ -527,17 +526,13 @@ __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxs
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     CFloat* vf = (CFloat*)p.views;
     constexpr int VSTRIDE = sizeof(DevView) / 4;
-    constexpr int64_t kChunk = 16384;
     const int tile_w = p.tile_w, tile_pix = p.tile_w * p.tile_h;
     while (true) {
         unsigned long long qv = 0;
         if (lane == 0) qv = atomicAdd(p.queue, 1ull);
         qv = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(qv >> 32)) << 32) |
              (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qv);
-        const int64_t q = (int64_t)qv;
-        const int64_t tile = p.sample_stride > 1
-                                 ? (q / kChunk) * kChunk * p.sample_stride + q % kChunk
-                                 : q;
+        const int64_t tile = (int64_t)qv;
         if (tile >= p.total_tiles) break;
         // view lookup: last view whose tile_offset <= tile (scalar loads)
         int lo = 0, hi = p.nviews - 1;
@@ -811,6 +806,146 @@ __global__ void adjoint_reduce_kernel(DevCloud c, DevView v, const int32_t* __re
     g_rgb[3 * (int64_t)key + 2] += c2;
 }
 
+// ------------------------------------------------------------------------------------------
+// Device tree build. The shape (segments, left counts, child refs) comes from the host; the
+// device keeps three lists of point ids, each sorted by one coordinate with ties by id, and per
+// level: picks each node's split axis from its tight box (the ends of the three lists), marks
+// the first `nl` points of the axis list as left, and stable-partitions all three lists, which
+// keeps every child segment sorted along every axis (so child boxes are again list ends).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ordered_key(float f) {
+    uint32_t u = __float_as_uint(f);
+    if (u == 0x80000000u) u = 0u;  // -0 sorts with +0
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float coord(const float4* p, int32_t id, int d) {
+    const float4 v = p[id];
+    return d == 0 ? v.x : (d == 1 ? v.y : v.z);
+}
+
+__global__ void build_keys_kernel(const float4* __restrict__ p, int64_t n, int d,
+                                  uint32_t* __restrict__ keys, int32_t* __restrict__ ids) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = ordered_key(coord(p, (int32_t)i, d));
+    ids[i] = (int32_t)i;
+}
+
+struct BuildLists {
+    const int32_t* l[3];
+};
+
+__global__ void build_axis_kernel(const int64_t* __restrict__ seg_b,
+                                  const int64_t* __restrict__ seg_e, int node0, int count,
+                                  const float4* __restrict__ p, BuildLists L,
+                                  int8_t* __restrict__ axis) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int node = node0 + i;
+    const int64_t b = seg_b[node], e = seg_e[node];
+    float ext[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ext[d] = coord(p, L.l[d][e - 1], d) - coord(p, L.l[d][b], d);
+    int dim = 0;
+    if (ext[1] > ext[dim]) dim = 1;
+    if (ext[2] > ext[dim]) dim = 2;
+    axis[node] = (int8_t)dim;
+}
+
+__global__ void build_flag_kernel(int64_t n, const int32_t* __restrict__ owner,
+                                  const int64_t* __restrict__ seg_b,
+                                  const int32_t* __restrict__ nl, const int8_t* __restrict__ axis,
+                                  BuildLists L, int32_t* __restrict__ flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int node = owner[i];
+    if (node < 0) return;
+    const int32_t* list = L.l[axis[node]];
+    flag[list[i]] = (i - seg_b[node] < nl[node]) ? 1 : 0;
+}
+
+__global__ void build_isleft_kernel(int64_t n, const int32_t* __restrict__ owner,
+                                    const int32_t* __restrict__ list,
+                                    const int32_t* __restrict__ flag, int32_t* __restrict__ il) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    il[i] = owner[i] >= 0 ? flag[list[i]] : 0;
+}
+
+__global__ void build_scatter_kernel(int64_t n, const int32_t* __restrict__ owner,
+                                     const int64_t* __restrict__ seg_b,
+                                     const int32_t* __restrict__ nl,
+                                     const int32_t* __restrict__ list,
+                                     const int32_t* __restrict__ il,
+                                     const int32_t* __restrict__ pl, int32_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int node = owner[i];
+    if (node < 0) {
+        out[i] = list[i];
+        return;
+    }
+    const int64_t b = seg_b[node];
+    const int64_t lc = (int64_t)pl[i] - pl[b];
+    const int64_t dst = il[i] ? b + lc : b + nl[node] + (i - b - lc);
+    out[dst] = list[i];
+}
+
+__global__ void build_box_kernel(int node0, int count, const int64_t* __restrict__ seg_b,
+                                 const int64_t* __restrict__ seg_e,
+                                 const int32_t* __restrict__ nl, const float4* __restrict__ p,
+                                 BuildLists L, RmNode* __restrict__ nodes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int node = node0 + i;
+    const int64_t b = seg_b[node], e = seg_e[node], m = b + nl[node];
+    const int64_t cb[2] = {b, m}, ce[2] = {m, e};
+    RmNode& nd = nodes[node];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            nd.box[s][d] = coord(p, L.l[d][cb[s]], d);
+            nd.box[s][3 + d] = coord(p, L.l[d][ce[s] - 1], d);
+        }
+}
+
+__global__ void build_leafroot_kernel(int64_t n, const float4* __restrict__ p, BuildLists L,
+                                      RmNode* __restrict__ nodes) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int d = 0; d < 3; ++d) {
+        nodes[0].box[0][d] = coord(p, L.l[d][0], d);
+        nodes[0].box[0][3 + d] = coord(p, L.l[d][n - 1], d);
+    }
+}
+
+__global__ void build_owner_kernel(int64_t n, int32_t* __restrict__ owner,
+                                   const int64_t* __restrict__ seg_b,
+                                   const int32_t* __restrict__ nl,
+                                   const RmNode* __restrict__ nodes) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int node = owner[i];
+    if (node < 0) return;
+    const int s = (i - seg_b[node] < nl[node]) ? 0 : 1;
+    owner[i] = nodes[node].cnt[s] == 0 ? nodes[node].ref[s] : -1;
+}
+
+__global__ void build_fill_kernel(int64_t n, int32_t v, int32_t* __restrict__ a) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void build_pts_kernel(int64_t n, const float4* __restrict__ p,
+                                 const int32_t* __restrict__ list, float4* __restrict__ pts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t id = list[i];
+    const float4 v = p[id];
+    pts[i] = make_float4(v.x, v.y, v.z, __uint_as_float((uint32_t)id));
+}
+
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 }  // namespace
@@ -974,6 +1109,89 @@ hipError_t launch_adjoint_reduce(hipStream_t s, const DevCloud& c, const DevView
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(grid_for(npix, 256)), dim3(256), 0, s, c, v,
                        keys_sorted, vals_sorted, npix, grad_rgb, grad_depth, g_xyz, g_r, g_rgb);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Device tree build driver
+// ------------------------------------------------------------------------------------------
+#define RM_BUILD_CHECK(call)                        \
+    do {                                            \
+        hipError_t e_ = (call);                     \
+        if (e_ != hipSuccess) return e_;            \
+    } while (0)
+
+hipError_t build_tree_device(hipStream_t s, const float* xyzw, const RmShape& shape,
+                             const int64_t* d_seg_b, const int64_t* d_seg_e,
+                             const int32_t* d_nl, RmNode* d_nodes, float4* d_pts,
+                             BuildScratch& w) {
+    const int64_t n = shape.n;
+    const float4* p = reinterpret_cast<const float4*>(xyzw);
+    const unsigned g = grid_for(n, 256);
+    // three id lists, each sorted by one coordinate (stable: ties keep ascending id)
+    for (int d = 0; d < 3; ++d) {
+        hipLaunchKernelGGL(build_keys_kernel, dim3(g), dim3(256), 0, s, p, n, d, w.keys,
+                           w.ids);
+        RM_BUILD_CHECK(hipGetLastError());
+        size_t bytes = w.temp_bytes;
+        RM_BUILD_CHECK(hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, w.keys, w.keys_out,
+                                                          w.ids, w.list[d][0], (int)n, 0, 32, s));
+    }
+    int cur = 0;
+    auto lists = [&](int c) {
+        BuildLists L;
+        for (int d = 0; d < 3; ++d) L.l[d] = w.list[d][c];
+        return L;
+    };
+    if (shape.nodes[0].cnt[1] < 0) {  // a single leaf
+        hipLaunchKernelGGL(build_leafroot_kernel, dim3(1), dim3(1), 0, s, n, p, lists(cur),
+                           d_nodes);
+        RM_BUILD_CHECK(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(build_fill_kernel, dim3(g), dim3(256), 0, s, n, 0, w.owner);
+        RM_BUILD_CHECK(hipGetLastError());
+        const int nlev = (int)shape.level_start.size() - 1;
+        for (int l = 0; l < nlev; ++l) {
+            const int node0 = shape.level_start[l];
+            const int count = shape.level_start[l + 1] - node0;
+            const unsigned gn = grid_for(count, 256);
+            hipLaunchKernelGGL(build_axis_kernel, dim3(gn), dim3(256), 0, s, d_seg_b, d_seg_e,
+                               node0, count, p, lists(cur), w.axis);
+            RM_BUILD_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(build_flag_kernel, dim3(g), dim3(256), 0, s, n, w.owner, d_seg_b,
+                               d_nl, w.axis, lists(cur), w.flag);
+            RM_BUILD_CHECK(hipGetLastError());
+            for (int d = 0; d < 3; ++d) {
+                hipLaunchKernelGGL(build_isleft_kernel, dim3(g), dim3(256), 0, s, n, w.owner,
+                                   w.list[d][cur], w.flag, w.il);
+                RM_BUILD_CHECK(hipGetLastError());
+                size_t bytes = w.temp_bytes;
+                RM_BUILD_CHECK(hipcub::DeviceScan::ExclusiveSum(w.temp, bytes, w.il, w.pl,
+                                                                (int)n, s));
+                hipLaunchKernelGGL(build_scatter_kernel, dim3(g), dim3(256), 0, s, n, w.owner,
+                                   d_seg_b, d_nl, w.list[d][cur], w.il, w.pl,
+                                   w.list[d][1 - cur]);
+                RM_BUILD_CHECK(hipGetLastError());
+            }
+            cur = 1 - cur;
+            hipLaunchKernelGGL(build_box_kernel, dim3(gn), dim3(256), 0, s, node0, count,
+                               d_seg_b, d_seg_e, d_nl, p, lists(cur), d_nodes);
+            RM_BUILD_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(build_owner_kernel, dim3(g), dim3(256), 0, s, n, w.owner,
+                               d_seg_b, d_nl, d_nodes);
+            RM_BUILD_CHECK(hipGetLastError());
+        }
+    }
+    hipLaunchKernelGGL(build_pts_kernel, dim3(g), dim3(256), 0, s, n, p, w.list[0][cur], d_pts);
+    return hipGetLastError();
+}
+
+size_t build_temp_bytes(int64_t n) {
+    size_t a = 0, b = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (int32_t*)nullptr, (int32_t*)nullptr, (int)n, 0, 32);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int32_t*)nullptr, (int32_t*)nullptr,
+                                           (int)n);
+    return a > b ? a : b;
 }
 
 }  // namespace rm

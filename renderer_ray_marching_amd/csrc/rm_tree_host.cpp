@@ -10,6 +10,14 @@
 
 namespace {
 
+int64_t split_left_count(int64_t c, int L) {
+    int64_t half = c / 2;
+    int64_t nl = ((half + L - 1) / L) * L;
+    if (nl >= c) nl = c - 1;
+    if (nl < 1) nl = 1;
+    return nl;
+}
+
 struct Builder {
     const float* xyzw;
     int64_t n;
@@ -19,13 +27,7 @@ struct Builder {
     std::unordered_map<int64_t, int64_t> inner_memo;
 
     // Left part of a split of c > L points: half, rounded up to a multiple of L.
-    int64_t left_count(int64_t c) const {
-        int64_t half = c / 2;
-        int64_t nl = ((half + L - 1) / L) * L;
-        if (nl >= c) nl = c - 1;
-        if (nl < 1) nl = 1;
-        return nl;
-    }
+    int64_t left_count(int64_t c) const { return split_left_count(c, L); }
 
     // Number of inner nodes of a subtree over c points (the shape depends on counts only).
     int64_t inner_count(int64_t c) {
@@ -177,5 +179,73 @@ bool rm_build_host_tree(const float* xyzw, int64_t n, int leaf_max, RmHostTree& 
         out.pts[4 * i + 2] = p[2];
         std::memcpy(&out.pts[4 * i + 3], &o, sizeof(float));
     }
+    return true;
+}
+
+bool rm_tree_shape(int64_t n, int leaf_max, RmShape& out, std::string& err) {
+    if (n <= 0) {
+        err = "rm_tree_build: empty point set";
+        return false;
+    }
+    if (n > (int64_t)std::numeric_limits<int32_t>::max() - 1) {
+        err = "rm_tree_build: more than 2^31-2 points";
+        return false;
+    }
+    if (leaf_max < 1 || leaf_max > 64) {
+        err = "rm_tree_build: leaf_max must be in [1, 64]";
+        return false;
+    }
+    out = RmShape();
+    out.n = n;
+    out.leaf_max = leaf_max;
+    const float inf = std::numeric_limits<float>::infinity();
+    if (n <= leaf_max) {
+        RmNode r{};
+        for (int d = 0; d < 3; ++d) {
+            r.box[1][d] = inf;
+            r.box[1][3 + d] = -inf;
+        }
+        r.ref[0] = 0;
+        r.cnt[0] = (int32_t)n;
+        r.ref[1] = 0;
+        r.cnt[1] = -1;
+        out.nodes.push_back(r);
+        out.seg_b.push_back(0);
+        out.seg_e.push_back(n);
+        out.nl.push_back((int32_t)n);
+        out.level_start = {0, 1};
+        out.depth = 1;
+        return true;
+    }
+    std::vector<int32_t> level;
+    out.nodes.push_back(RmNode{});
+    out.seg_b.push_back(0);
+    out.seg_e.push_back(n);
+    level.push_back(0);
+    for (size_t qi = 0; qi < out.nodes.size(); ++qi) {
+        const int64_t b = out.seg_b[qi], e = out.seg_e[qi];
+        const int64_t nl = split_left_count(e - b, leaf_max);
+        out.nl.push_back((int32_t)nl);
+        const int64_t cb[2] = {b, b + nl}, ce[2] = {b + nl, e};
+        for (int s = 0; s < 2; ++s) {
+            const int64_t c = ce[s] - cb[s];
+            if (c <= leaf_max) {
+                out.nodes[qi].ref[s] = (int32_t)cb[s];
+                out.nodes[qi].cnt[s] = (int32_t)c;
+            } else {
+                out.nodes[qi].ref[s] = (int32_t)out.nodes.size();
+                out.nodes[qi].cnt[s] = 0;
+                out.nodes.push_back(RmNode{});
+                out.seg_b.push_back(cb[s]);
+                out.seg_e.push_back(ce[s]);
+                level.push_back(level[qi] + 1);
+            }
+        }
+    }
+    const int nlev = level.back() + 1;
+    out.depth = nlev;
+    out.level_start.assign(nlev + 1, 0);
+    for (int32_t l : level) out.level_start[l + 1]++;
+    for (int l = 0; l < nlev; ++l) out.level_start[l + 1] += out.level_start[l];
     return true;
 }

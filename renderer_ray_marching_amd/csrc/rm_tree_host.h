@@ -35,3 +35,20 @@ struct RmHostTree {
 // Build from n x 4 floats (stride 16 B). Returns false with a message on bad input.
 bool rm_build_host_tree(const float* xyzw, int64_t n, int leaf_max, RmHostTree& out,
                         std::string& err);
+
+// Shape of the tree for n points and leaf size L: it depends on the counts only (the split
+// keeps the first `nl` points of the node in the order of its split axis), so the host lays out
+// every inner node before any coordinate is read and the device build fills in the split axes,
+// the point order and the boxes. Inner nodes are numbered breadth first.
+struct RmShape {
+    int64_t n = 0;
+    int leaf_max = 0;
+    int depth = 0;
+    std::vector<RmNode> nodes;       // refs and counts set, boxes left for the device build
+    std::vector<int64_t> seg_b;      // per inner node: first position of its segment
+    std::vector<int64_t> seg_e;      // one past its last position
+    std::vector<int32_t> nl;         // points of its left child
+    std::vector<int32_t> level_start;  // inner nodes of level l: [level_start[l], level_start[l+1])
+};
+
+bool rm_tree_shape(int64_t n, int leaf_max, RmShape& out, std::string& err);

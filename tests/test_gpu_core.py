@@ -89,6 +89,30 @@ def test_knn_ties_duplicates_and_padding(rm_ctx):
     assert (idx == oi).all() and (d2 == od).all()
 
 
+@pytest.mark.parametrize("n,leaf", [(1, 16), (16, 16), (17, 16), (1000, 1), (50_000, 16),
+                                    (300_000, 32)])
+def test_device_and_host_builds_agree(rm_ctx, n, leaf, monkeypatch):
+    """The device build and the host build make the same partitions (same rule, same ties), so
+    every search result is identical; both equal the oracle."""
+    p = np.ascontiguousarray(scenes.room_scene(max(n, 2), seed=9).xyzw()[:n])
+    q = _queries(p, 5, 900) if n > 3 else p
+    t_dev = _tree(rm_ctx, p, leaf)
+    monkeypatch.setenv("RM_TREE_BUILD", "host")
+    t_host = _tree(rm_ctx, p, leaf)
+    monkeypatch.delenv("RM_TREE_BUILD")
+    assert t_dev.info() == t_host.info()
+    k = min(10, n)
+    _, i1, d1 = t_dev.SearchKNN(_dev(q), k)
+    _, i2, d2 = t_host.SearchKNN(_dev(q), k)
+    ru1, rc1 = t_dev.radii(0.05)
+    ru2, rc2 = t_host.radii(0.05)
+    rm_ctx.sync()
+    assert torch.equal(i1, i2) and torch.equal(d1, d2)
+    assert torch.equal(ru1, ru2) and torch.equal(rc1, rc2)
+    oi, od = ob.knn_brute(p, q, k) if n <= 50_000 else ob.RefTree(p, 64).knn(q, k)[:2]
+    assert (i1.cpu().numpy() == oi).all() and (d1.cpu().numpy() == od).all()
+
+
 def test_knn_rejects_bad_k(rm_ctx):
     p = scenes.box_scene(1000, seed=1).xyzw()
     t = _tree(rm_ctx, p)
