@@ -740,7 +740,9 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
                          "%.3f ms\n",
                          t.tile_w, t.tile_h, t.mode, t.warm, t.waves_per_block, t.blocks_per_cu,
                          t.refill, ms);
-        if (r == RM_OK && ms < best_ms) {
+        // a candidate must beat the incumbent by 2 % (timing noise would otherwise flip
+        // between equivalent settings from run to run)
+        if (r == RM_OK && ms < best_ms * 0.98f) {
             best_ms = ms;
             best = t;
         }
