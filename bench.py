@@ -306,7 +306,7 @@ def diff_leg(api, scenes, torch, steps):
 
 
 def cpu_baseline(cloud, views, radii):
-    """Oracle march (C++/OpenMP restatement of the reference algorithm) over every 256th pixel
+    """Oracle march (C++/OpenMP restatement of the reference algorithm) over every 16th pixel
     of every view, on all host cores available to this process."""
     from oracle import binding as ob
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -315,7 +315,7 @@ def cpu_baseline(cloud, views, radii):
     tree = ob.RefTree(cloud.xyzw(), 64)
     log(f"cpu baseline: oracle tree ({time.time() - t0:.1f}s)")
     col = cloud.colors4()
-    stride = 32
+    stride = 16
     rays = 0
     tcpu = 0.0
     for v in views:
