@@ -53,6 +53,10 @@ def lib():
                               C.c_int]
         L.orc_knn_brute.argtypes = [_f, C.c_int64, _f, C.c_int64, C.c_int, _i, _f, C.c_int]
         L.orc_radii.argtypes = [C.c_void_p, C.c_float, _f, _f, C.POINTER(Counters), C.c_int]
+        L.orc_subsample_indices.argtypes = [C.c_int64, _i]
+        L.orc_subsample_indices.restype = None
+        L.orc_mt19937_10000th.restype = C.c_uint32
+        L.orc_mt19937_10000th.argtypes = []
         L.orc_pane_distance.restype = C.c_float
         L.orc_pane_distance.argtypes = [C.c_int, C.c_float]
         L.orc_render.argtypes = [C.c_void_p, _f, _f, _f, C.c_float, C.c_int, C.c_int,
@@ -176,6 +180,17 @@ def knn_brute(pts, q, k, nthreads=0):
     d2 = np.empty((q.shape[0], k), np.float32)
     lib().orc_knn_brute(p, p.shape[0], q, q.shape[0], k, idx, d2, nthreads)
     return idx, d2
+
+
+def subsample_indices(n):
+    """Order of main.cu:75-87's -s subsample (its first mp entries are the subsample)."""
+    out = np.empty(n, np.int32)
+    lib().orc_subsample_indices(n, out)
+    return out
+
+
+def mt19937_10000th():
+    return int(lib().orc_mt19937_10000th())
 
 
 def pane_distance(W, fov):

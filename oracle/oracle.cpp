@@ -11,6 +11,8 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <numeric>
+#include <random>
 #include <vector>
 
 #ifdef _OPENMP
@@ -412,6 +414,24 @@ void orc_radii(const void* tree, float max_radius, float* r_unclamped, float* r_
         }
     }
     if (cnt) *cnt = total;
+}
+
+// main.cu:75-87: indices 0..n-1 shuffled by std::shuffle with std::mt19937 seeded 42; the
+// subsample is the first mp of them. Same standard-library calls as the reference, so equal to
+// its subsample when built against the same libstdc++.
+void orc_subsample_indices(int64_t n, int32_t* out) {
+    std::vector<int> idx((size_t)n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::mt19937 g(42);
+    std::shuffle(idx.begin(), idx.end(), g);
+    std::memcpy(out, idx.data(), sizeof(int) * (size_t)n);
+}
+
+// The standard fixes the 10000th output of a default-constructed std::mt19937 (seed 5489).
+uint32_t orc_mt19937_10000th(void) {
+    std::mt19937 g;
+    g.discard(9999);
+    return (uint32_t)g();
 }
 
 float orc_pane_distance(int W, float fov_radians) {

@@ -23,6 +23,9 @@
  *   orc_render      single-sphere analytic depth, tiny scenes vs brute-force SDF (kNN-10 = all
  *                   points), miss/background, start-inside case
  *   orc_render_grad central finite differences of orc_render_loss_fixed (fp64)
+ *   orc_subsample_indices  is a permutation; the generator's standard-mandated 10000th value;
+ *                   the shuffle itself is libstdc++'s (parity with the reference's GCC 9 build
+ *                   unpinned)
  *
  * All float arithmetic is IEEE single precision, built with -ffp-contract=off so no
  * multiply-add is fused; the same expression order is used by the HIP kernels.
@@ -67,6 +70,12 @@ void orc_knn_brute(const float* xyzw, int64_t n, const float* q_xyzw, int64_t m,
  * clamped copy: r > max_radius ? max_radius : r, only when max_radius > 0. */
 void orc_radii(const void* tree, float max_radius, float* r_unclamped, float* r_clamped,
                orc_counters* cnt, int nthreads);
+
+/* Subsample order of main.cu:75-87: 0..n-1 shuffled by std::shuffle(std::mt19937(42));
+ * the -s subsample is the first mp entries. */
+void orc_subsample_indices(int64_t n, int32_t* out);
+/* 10000th output of a default-seeded std::mt19937 (4123659995 by the C++ standard). */
+uint32_t orc_mt19937_10000th(void);
 
 /* Host-side pane distance W / (2 tan(fov/2)) (src/ray_marching.cu:74), float, libm tanf. */
 float orc_pane_distance(int W, float fov_radians);

@@ -43,3 +43,28 @@ def test_ctx_create_without_device_fails_cleanly():
         pytest.skip("a device is present")
     h = C.c_void_p()
     assert _abi.lib().rm_ctx_create(0, C.byref(h)) == _abi.RM_ERR_INVALID
+
+
+def _declared_host():
+    src = open(os.path.join(ROOT, "include", "rm_host.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int|int64_t|void)\s+(rmh_\w+)\(", src, re.M)))
+
+
+def test_host_header_declares_the_binding_and_library_exports_it():
+    from renderer_ray_marching_amd import host_io
+    assert set(_declared_host()) == set(host_io.SIGNATURES)
+    L = host_io.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", host_io.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (rmh_\w+)", out))
+    assert set(_declared_host()) <= exported
+    for name in _declared_host():
+        assert hasattr(L, name), name
+
+
+def test_host_view_spec_layout():
+    import ctypes as C
+    from renderer_ray_marching_amd import host_io
+    # const char* key; int32 split, has_ply_path; const char* ply_path; float pose[16], calib[16];
+    # int32 width, height; float fov
+    assert C.sizeof(host_io.rmh_view_spec) == 8 + 4 + 4 + 8 + 64 + 64 + 4 + 4 + 4 + 4

@@ -312,3 +312,12 @@ def test_march_tiny_scene_equals_brute_sdf():
         assert hit[lin] == res
         if res >= 0:
             assert rgba[lin, 3] == tt
+
+
+def test_subsample_order():
+    # the generator is pinned by the value the C++ standard mandates; the order is a permutation
+    assert ob.mt19937_10000th() == 4123659995
+    for n in (1, 2, 17, 70000):
+        p = ob.subsample_indices(n)
+        assert np.array_equal(np.sort(p), np.arange(n))
+    assert np.array_equal(ob.subsample_indices(1000), ob.subsample_indices(1000))
