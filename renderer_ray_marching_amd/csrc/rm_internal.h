@@ -19,6 +19,14 @@ constexpr int kMarchK = 10;
 // Largest tree depth the traversals accept (stack bound); the host refuses deeper trees.
 constexpr int kMaxDepth = 60;
 constexpr int kWaveStack = 64;
+// Leaves queued per round of the batched march's packet traversal.
+#ifndef RM_LEAF_QUEUE
+#define RM_LEAF_QUEUE 8
+#endif
+constexpr int kLeafQueue = RM_LEAF_QUEUE;
+// Extra float4 entries after the leaf-ordered points: leaf scans read points in groups of 4 and
+// may read up to 3 entries past the last leaf.
+constexpr int kPtsPad = 4;
 
 // Device view of a tree + cloud, passed by value to kernels.
 struct DevTree {

@@ -213,6 +213,42 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
     }
 }
 
+// Same result as knn_packet, in rounds: expand inner nodes (reading only the k-th distance)
+// until the stack is empty or kLeafQueue leaves are queued, then scan the queued leaves in
+// order, re-testing each leaf's box against the current k-th distance. The list is modified in
+// the scan loop only, so it keeps its registers across the node expansions. Leaves are queued
+// in the near-first order of the depth-first walk; bounds tightened by a round prune the
+// entries still on the stack when they are expanded.
+template <int K, bool NODUP = false>
+__device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz, bool active,
+                                  TopK<K>& tk, int* wstk, int* lq, int* err) {
+    CFloat* nf = (CFloat*)t.nodes;
+    CFloat* pf = (CFloat*)t.pts;
+    int sp = 0;
+    packet_push_children<K>(nf, 0, qx, qy, qz, active, tk, wstk, sp, err);
+    while (sp > 0) {
+        int nq = 0;
+        while (sp > 0 && nq < kLeafQueue) {
+            --sp;
+            const int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
+            if (e & 1)
+                lq[nq++] = e;
+            else
+                packet_push_children<K>(nf, e >> 1, qx, qy, qz, active, tk, wstk, sp, err);
+        }
+        for (int i = 0; i < nq; ++i) {
+            const int e = __builtin_amdgcn_readfirstlane(lq[i]);
+            const int ni = e >> 2, s = (e >> 1) & 1;
+            CFloat* f = nf + (int64_t)ni * 16;
+            CInt* fi = (CInt*)f;
+            CFloat* bx = f + 6 * s;
+            const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
+            if (__builtin_amdgcn_ballot_w64(active & (md <= tk.worst_d2())) == 0) continue;
+            packet_scan_leaf<K, NODUP>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Per-lane traversal: every lane walks its own path with its own stack, held in LDS as
 // column `lane` of a [depth][threads] array (consecutive lanes hit consecutive banks).
@@ -520,6 +556,7 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
 template <int MODE, bool WARM, int WPB>
 __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
+    __shared__ int lq[WPB][kLeafQueue];
     extern __shared__ int lstk[];
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
@@ -599,9 +636,11 @@ __global__ __launch_bounds__(WPB * 64) void batch_kernel(BatchParams p, int maxs
                 // a fresh lane's pads (index 0xffffffff) never match a real candidate, so the
                 // duplicate check is harmless for it and one traversal serves both kinds
                 if (WARM && __builtin_amdgcn_ballot_w64(warm_lane) != 0)
-                    knn_packet<kMarchK, true>(p.cloud.tree, px, py, pz, has, tk, wstk[w], p.err);
+                    knn_packet_rounds<kMarchK, true>(p.cloud.tree, px, py, pz, has, tk, wstk[w],
+                                                     lq[w], p.err);
                 else
-                    knn_packet<kMarchK>(p.cloud.tree, px, py, pz, has, tk, wstk[w], p.err);
+                    knn_packet_rounds<kMarchK>(p.cloud.tree, px, py, pz, has, tk, wstk[w], lq[w],
+                                               p.err);
             } else if (has) {
                 if (warm_lane)
                     knn_lane<kMarchK, true>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
