@@ -273,13 +273,12 @@ int tree_build_host(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_
         for (size_t i = 0; i < ht.level.size(); ++i) order[fillp[ht.level[i]]++] = (int32_t)i;
     }
     if (hipMalloc(&t->nodes, sizeof(RmNode) * ht.nodes.size()) != hipSuccess ||
-        hipMalloc(&t->pts, sizeof(float4) * ((size_t)n + rm::kPtsPad)) != hipSuccess ||
+        hipMalloc(&t->pts, sizeof(float4) * (size_t)n) != hipSuccess ||
         hipMalloc(&t->level_order, sizeof(int32_t) * std::max<size_t>(1, order.size())) !=
             hipSuccess) {
         (void)hipGetLastError();
         return fail(ctx, RM_ERR_NOMEM, "rm_tree_build: device allocation failed");
     }
-    RM_HIP(ctx, hipMemsetAsync(t->pts + n, 0, sizeof(float4) * rm::kPtsPad, ctx->stream));
     RM_HIP(ctx, hipMemcpyAsync(t->nodes, ht.nodes.data(), sizeof(RmNode) * ht.nodes.size(),
                                hipMemcpyHostToDevice, ctx->stream));
     RM_HIP(ctx, hipMemcpyAsync(t->pts, ht.pts.data(), sizeof(float4) * (size_t)n,
@@ -306,12 +305,11 @@ int tree_build_device(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, r
     std::vector<int32_t> order(nn);
     for (size_t i = 0; i < nn; ++i) order[i] = (int32_t)i;  // breadth-first = level order
     if (hipMalloc(&t->nodes, sizeof(RmNode) * nn) != hipSuccess ||
-        hipMalloc(&t->pts, sizeof(float4) * ((size_t)n + rm::kPtsPad)) != hipSuccess ||
+        hipMalloc(&t->pts, sizeof(float4) * (size_t)n) != hipSuccess ||
         hipMalloc(&t->level_order, sizeof(int32_t) * nn) != hipSuccess) {
         (void)hipGetLastError();
         return fail(ctx, RM_ERR_NOMEM, "rm_tree_build: device allocation failed");
     }
-    RM_HIP(ctx, hipMemsetAsync(t->pts + n, 0, sizeof(float4) * rm::kPtsPad, ctx->stream));
     const size_t temp = rm::build_temp_bytes(n);
     const size_t n4 = align_up(sizeof(int32_t) * (size_t)n);
     const size_t total = align_up(sizeof(float4) * (size_t)n) + 15 * n4 + align_up(nn) +

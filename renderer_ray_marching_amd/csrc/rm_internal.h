@@ -24,9 +24,7 @@ constexpr int kWaveStack = 64;
 #define RM_LEAF_QUEUE 8
 #endif
 constexpr int kLeafQueue = RM_LEAF_QUEUE;
-// Extra float4 entries after the leaf-ordered points: leaf scans read points in groups of 4 and
-// may read up to 3 entries past the last leaf.
-constexpr int kPtsPad = 4;
+
 
 // Device view of a tree + cloud, passed by value to kernels.
 struct DevTree {
