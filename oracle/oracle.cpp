@@ -140,10 +140,11 @@ RefTree* build_ref_tree(const float* xyzw, int64_t n, int leaf_max) {
 }
 
 // Squared distance of kdtree_cuda_3d_index.cu:186-190 (dot of the float4 difference, w = 0),
-// evaluated without contraction as ((dx*dx + dy*dy) + dz*dz).
+// accumulated left to right with fused multiply-adds: fma(dz, dz, fma(dy, dy, dx * dx)).
+// (std::fma on floats; built with -mfma it is one instruction.)
 inline float dist2(float px, float py, float pz, float qx, float qy, float qz) {
     float dx = px - qx, dy = py - qy, dz = pz - qz;
-    return (dx * dx + dy * dy) + dz * dz;
+    return std::fma(dz, dz, std::fma(dy, dy, dx * dx));
 }
 
 inline bool key_less(float da, int32_t ia, float db, int32_t ib) {

@@ -28,7 +28,8 @@
  *                   unpinned)
  *
  * All float arithmetic is IEEE single precision, built with -ffp-contract=off so no
- * multiply-add is fused; the same expression order is used by the HIP kernels.
+ * multiply-add is fused implicitly; the squared distance uses explicit fused multiply-adds,
+ * fma(dz, dz, fma(dy, dy, dx * dx)). The HIP kernels use the same expressions.
  */
 #ifndef RM_ORACLE_H
 #define RM_ORACLE_H

@@ -21,7 +21,7 @@ constexpr int kMaxDepth = 60;
 constexpr int kWaveStack = 64;
 // Leaves queued per round of the batched march's packet traversal.
 #ifndef RM_LEAF_QUEUE
-#define RM_LEAF_QUEUE 8
+#define RM_LEAF_QUEUE 2
 #endif
 constexpr int kLeafQueue = RM_LEAF_QUEUE;
 

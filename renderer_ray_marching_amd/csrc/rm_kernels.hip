@@ -2,8 +2,9 @@
 // batched sphere march, tree refit, and the differentiable forward/adjoint.
 //
 // Arithmetic contract (shared with the CPU oracle, which restates the reference):
-//   squared distance  ((dx*dx + dy*dy) + dz*dz), dx = p.x - q.x      (no fused multiply-add:
-//                     the library is built with -ffp-contract=off)
+//   squared distance  fma(dz, dz, fma(dy, dy, dx * dx)), dx = p.x - q.x (explicit fused
+//                     multiply-adds; everything else is built with -ffp-contract=off and
+//                     evaluated exactly as written)
 //   ordering          ascending (d2, original index): the 64-bit key (bits(d2) << 32 | index)
 //                     compares exactly like that pair because d2 >= 0.
 //   pruning           a child box is visited iff its mindist <= the current k-th d2; mindist
@@ -28,23 +29,25 @@ __device__ __forceinline__ uint64_t make_key(float d2, uint32_t idx) {
 __device__ __forceinline__ float key_d2(uint64_t k) { return __uint_as_float(uint32_t(k >> 32)); }
 __device__ __forceinline__ uint32_t key_idx(uint64_t k) { return uint32_t(k); }
 
-// Squared distance of kdtree_cuda_3d_index.cu:186-190 without contraction.
+// Squared distance of kdtree_cuda_3d_index.cu:186-190: the dot product of the difference,
+// accumulated left to right with fused multiply-adds (the form a contracting compiler gives
+// dot(d, d); the w term is 0 and drops out exactly): fma(dz, dz, fma(dy, dy, dx * dx)).
 __device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy,
                                        float qz) {
     float dx = px - qx;
     float dy = py - qy;
     float dz = pz - qz;
-    return (dx * dx + dy * dy) + dz * dz;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-// Lower bound of dist2(p, q) over p in the box [l, h] (per axis |p - q| >= the gap, and
-// float subtraction, squaring and addition are monotone).
+// Lower bound of dist2(p, q) over p in the box [l, h]: per axis |p - q| >= the gap, and the
+// rounded subtraction, product and fused multiply-adds are monotone in their operands.
 __device__ __forceinline__ float mindist(float lx, float ly, float lz, float hx, float hy,
                                          float hz, float qx, float qy, float qz) {
     float ex = fmaxf(fmaxf(lx - qx, qx - hx), 0.0f);
     float ey = fmaxf(fmaxf(ly - qy, qy - hy), 0.0f);
     float ez = fmaxf(fmaxf(lz - qz, qz - hz), 0.0f);
-    return (ex * ex + ey * ey) + ez * ez;
+    return __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
 }
 
 // Ascending top-K of 64-bit keys held in registers (fully unrolled).

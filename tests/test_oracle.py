@@ -11,6 +11,7 @@ import os
 import numpy as np
 import pytest
 
+import fp32
 from oracle import binding as ob
 from renderer_ray_marching_amd import scenes
 
@@ -32,8 +33,7 @@ def test_brute_matches_numpy_definition():
     p[7, :3] = p[3, :3]          # an exact duplicate: tie broken by index
     idx, d2 = ob.knn_brute(p, q, 7)
     for j in range(q.shape[0]):
-        diff = p[:, :3] - q[j, :3]
-        dd = (diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1]) + diff[:, 2] * diff[:, 2]
+        dd = fp32.dist2(p, q[j])
         order = np.lexsort((np.arange(p.shape[0]), dd))[:7]
         assert (idx[j] == order).all()
         assert (d2[j] == dd[order]).all()
@@ -183,13 +183,16 @@ def test_oracle_against_flann_fixture(name):
     idx_f, d2_f = f["idx"], f["d2"]
     t = ob.RefTree(p, 64)
     idx, d2, _ = t.knn(q, k)
-    # rank by rank, the exact list is never worse than any k points FLANN can return
-    assert (d2 <= d2_f).all()
-    # FLANN's distances are the true distances of the points it names
+    # FLANN's distances are the true distances of the points it names (its own expression,
+    # ((dx*dx + dy*dy) + dz*dz) without fused multiply-adds, flann/algorithms/dist.h:172-175)
     diff = p[idx_f, :3] - q[:, None, :3]
     dd = (diff[..., 0] * diff[..., 0] + diff[..., 1] * diff[..., 1]) + diff[..., 2] * diff[..., 2]
     assert (dd == d2_f).all()
-    same = (d2 == d2_f).all(1)
+    # rank by rank, the exact list is never worse than the k points FLANN returns, both measured
+    # with the oracle's distance
+    ours_f = np.sort(fp32.dist2(p[idx_f], q[:, None, :]), axis=1)
+    assert (d2 <= ours_f).all()
+    same = (d2 == ours_f).all(1)
     assert same.mean() > (0.98 if k == 2 else 0.9)
     # where the distance lists agree, the index sets agree too (up to ties' order)
     for j in np.nonzero(same)[0][:2000]:
@@ -207,8 +210,9 @@ def test_flann_live_cross_check():
     for k in (2, 10):
         idx_f, d2_f = ob.flann_knn(p, q, k)
         idx, d2, _ = t.knn(q, k)
-        assert (d2 <= d2_f).all()
-        assert (d2 == d2_f).all(1).mean() > (0.98 if k == 2 else 0.9)
+        ours_f = np.sort(fp32.dist2(p[idx_f], q[:, None, :]), axis=1)
+        assert (d2 <= ours_f).all()
+        assert (d2 == ours_f).all(1).mean() > (0.98 if k == 2 else 0.9)
 
 
 # ------------------------------------------------------------------------------------------
@@ -297,8 +301,7 @@ def test_march_tiny_scene_equals_brute_sdf():
         res = -1
         for _ in range(128):
             pnt = d * tt
-            diff = pts - pnt
-            dd = (diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1]) + diff[:, 2] * diff[:, 2]
+            dd = fp32.dist2(pts, pnt)
             order = np.lexsort((np.arange(8), dd))
             s_all = np.sqrt(dd[order]) - radii[order]
             j = int(np.argmin(s_all))
