@@ -131,7 +131,8 @@ typedef struct rm_tune {
     int32_t mode;             /* 0 = packet traversal (wave-uniform), 1 = per-lane traversal */
     int32_t warm;             /* 1 = warm-start bound from the previous step's neighbours */
     int32_t waves_per_block;  /* 1, 2 or 4 */
-    int32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu (0 = 16 / waves_per_block) */
+    int32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu (0 = 32 / waves_per_block,
+                                 8 wavefronts per SIMD) */
     int32_t refill;           /* lanes whose ray ended take the tile's next pixels once at
                                  least this many (1..64) are idle */
 } rm_tune;

@@ -585,8 +585,9 @@ int batch_layout(rm_batch* b, int view_stride = 1) {
     RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     int bpc = b->tune.blocks_per_cu;
     if (bpc == 0) {
-        // occupancy-derived persistent grid: as many resident blocks as fit
-        bpc = std::max(1, 16 / b->tune.waves_per_block);
+        // occupancy-derived persistent grid: as many resident blocks as fit (the kernel is
+        // built for 8 wavefronts per SIMD, 32 per CU)
+        bpc = std::max(1, 32 / b->tune.waves_per_block);
     }
     b->blocks = ctx->num_cus * bpc;
     return RM_OK;
@@ -772,7 +773,7 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         consider(t);
     }
     cur = best;
-    for (int bpc : {4, 16}) {
+    for (int bpc : {4, 8}) {
         rm_tune t = cur;
         t.blocks_per_cu = bpc;
         consider(t);
