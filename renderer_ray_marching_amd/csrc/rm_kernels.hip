@@ -480,11 +480,12 @@ __device__ __forceinline__ void sdf10(const DevCloud& c, const TopK<kMarchK>& tk
 // points not already listed (insert_nodup), and its result is the exact top 10. Needs n >= 10.
 __device__ __forceinline__ void warm_seed(const DevCloud& c, TopK<kMarchK>& tk, float qx,
                                           float qy, float qz) {
+    const float* xyzr = (const float*)c.xyzr;
 #pragma unroll
     for (int j = 0; j < kMarchK; ++j) {
-        uint32_t id = key_idx(tk.key[j]);
-        float4 p = c.xyzr[id];
-        tk.key[j] = make_key(dist2(p.x, p.y, p.z, qx, qy, qz), id);
+        const uint32_t id = key_idx(tk.key[j]);
+        const float* q = xyzr + 4 * (size_t)id;   // x, y, z only: the radius is not needed
+        tk.key[j] = make_key(dist2(q[0], q[1], q[2], qx, qy, qz), id);
     }
     static_assert(kMarchK == 10, "sort10 sorts ten keys");
     sort10(tk.key);
@@ -602,12 +603,13 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
         const int64_t lt = tile - tile_off;
         const int x0 = (int)(lt % tiles_x) * tile_w;
         const int y0 = (int)(lt / tiles_x) * p.tile_h;
+        // camera origin: the same for every lane of the tile (scalar registers)
+        const float ox = V[12], oy = V[13], oz = V[14];
         int next = 0;                      // wave-uniform: next unclaimed pixel of the tile
         bool has = false;
-        Ray r = {0.f, 0.f, 0.f, 0.f, 0.f, -1.f, 0.f, 0.f};
-        float t = 0.0f;
-        int steps = 0;
-        int64_t out = 0;
+        // per-lane ray state: direction, far limit, distance marched, steps, pixel in the view
+        float dx = 0.f, dy = 0.f, dz = -1.f, tfar = 0.f, t = 0.f;
+        int steps = 0, pix = 0;
         TopK<kMarchK> tk;
         tk.init(kPadKey);
         while (true) {
@@ -618,10 +620,14 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                 if (!has && my < tile_pix) {
                     const int x = x0 + my % tile_w, y = y0 + my / tile_w;
                     if (x < W && y < H) {
-                        r = make_ray([&](int i) { return V[i]; }, pane, W, H, x, y);
+                        const Ray r = make_ray([&](int i) { return V[i]; }, pane, W, H, x, y);
+                        dx = r.dx;
+                        dy = r.dy;
+                        dz = r.dz;
+                        tfar = r.tfar;
                         t = r.tnear;
                         steps = 0;
-                        out = pix_off + (int64_t)y * W + x;
+                        pix = y * W + x;
                         has = true;
                     }
                 }
@@ -631,9 +637,9 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                 if (next >= tile_pix) break;
                 continue;
             }
-            const float px = r.ox + r.dx * t;
-            const float py = r.oy + r.dy * t;
-            const float pz = r.oz + r.dz * t;
+            const float px = ox + dx * t;
+            const float py = oy + dy * t;
+            const float pz = oz + dz * t;
             const bool warm_lane = WARM && has && steps > 0;
             if (!has)
                 tk.init(0ull);
@@ -644,12 +650,8 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
             if (MODE == 0) {
                 // a fresh lane's pads (index 0xffffffff) never match a real candidate, so the
                 // duplicate check is harmless for it and one traversal serves both kinds
-                if (WARM && __builtin_amdgcn_ballot_w64(warm_lane) != 0)
-                    knn_packet_rounds<kMarchK, true>(p.cloud.tree, px, py, pz, has, tk, wstk[w],
-                                                     lq[w], p.err);
-                else
-                    knn_packet_rounds<kMarchK>(p.cloud.tree, px, py, pz, has, tk, wstk[w], lq[w],
-                                               p.err);
+                knn_packet_rounds<kMarchK, WARM>(p.cloud.tree, px, py, pz, has, tk, wstk[w], lq[w],
+                                                 p.err);
             } else if (has) {
                 if (warm_lane)
                     knn_lane<kMarchK, true>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
@@ -665,8 +667,9 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                 t = t + s;
                 ++steps;
                 const bool hitnow = s < kMinDist;
-                if (hitnow || t > r.tfar || steps >= kMaxSteps) {
-                    write_pixel(p.cloud, p.out, out, hitnow ? (int)id : -1, hitnow ? t : 0.0f);
+                if (hitnow || t > tfar || steps >= kMaxSteps) {
+                    write_pixel(p.cloud, p.out, pix_off + pix, hitnow ? (int)id : -1,
+                                hitnow ? t : 0.0f);
                     has = false;
                 }
             }
