@@ -180,8 +180,12 @@ __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, flo
         float d[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) d[u] = dist2(x[u], y[u], z[u], qx, qy, qz);
+        // d2 <= the k-th d2 is necessary for key < the k-th key: a 32-bit test per candidate,
+        // and the key is formed and compared only when some lane passes it
 #pragma unroll
-        for (int u = 0; u < 4; ++u) tk.template add<NODUP>(make_key(d[u], id[u]));
+        for (int u = 0; u < 4; ++u)
+            if (__builtin_amdgcn_ballot_w64(d[u] <= tk.worst_d2()) != 0)
+                tk.template add<NODUP>(make_key(d[u], id[u]));
     }
     for (; j < cnt; ++j) {
         float d = dist2(p[4 * j], p[4 * j + 1], p[4 * j + 2], qx, qy, qz);
@@ -463,10 +467,19 @@ __device__ __forceinline__ void sdf10(const DevCloud& c, const TopK<kMarchK>& tk
     for (int j = 0; j < kMarchK; ++j) {
         uint32_t id = key_idx(tk.key[j]);
         if (id < (uint32_t)c.tree.n) {
-            float s = sqrtf(key_d2(tk.key[j])) - c.xyzr[id].w;
-            if (s < best) {
-                best = s;
-                bi = id;
+            const float d2 = key_d2(tk.key[j]);
+            const float r = c.xyzr[id].w;
+            // The hardware square root is within 1 ulp; scaled by (1 - 2^-20) it is a lower bound
+            // of sqrtf(d2) (0 for flushed tiny inputs is one too). Subtraction rounds
+            // monotonically, so lb - r >= best implies sqrtf(d2) - r >= best: the entry cannot
+            // become the strict minimum and its correctly rounded square root is skipped.
+            const float lb = __builtin_amdgcn_sqrtf(d2) * (1.0f - 0x1p-20f);
+            if (lb - r < best) {
+                const float s = sqrtf(d2) - r;
+                if (s < best) {
+                    best = s;
+                    bi = id;
+                }
             }
         }
     }
