@@ -131,9 +131,11 @@ __device__ __forceinline__ int child_entry(int ni, int s, int ref, int cnt) {
 // leaf points are read with scalar loads (the node index is wave-uniform) and broadcast to
 // every lane. The stack is wave-uniform and lives in LDS (one row of kWaveStack per wave).
 // ------------------------------------------------------------------------------------------
+// amask: ballot of the active lanes (a wave-uniform 64-bit mask). A compare's ballot masked with
+// it needs no per-lane boolean (ballot of (active && cond) would materialise one in a VGPR).
 template <int K>
 __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float qx, float qy,
-                                                     float qz, bool active, const TopK<K>& tk,
+                                                     float qz, uint64_t amask, const TopK<K>& tk,
                                                      int* wstk, int& sp, int* err) {
     CFloat* f = nf + (int64_t)ni * 16;
     CInt* fi = (CInt*)f;
@@ -141,8 +143,8 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
     const float w = tk.worst_d2();
     const float m0 = mindist(f[0], f[1], f[2], f[3], f[4], f[5], qx, qy, qz);
     const float m1 = mindist(f[6], f[7], f[8], f[9], f[10], f[11], qx, qy, qz);
-    const uint64_t b0 = c0 >= 0 ? __builtin_amdgcn_ballot_w64(active & (m0 <= w)) : 0ull;
-    const uint64_t b1 = c1 >= 0 ? __builtin_amdgcn_ballot_w64(active & (m1 <= w)) : 0ull;
+    const uint64_t b0 = c0 >= 0 ? __builtin_amdgcn_ballot_w64(m0 <= w) & amask : 0ull;
+    const uint64_t b1 = c1 >= 0 ? __builtin_amdgcn_ballot_w64(m1 <= w) & amask : 0ull;
     if (sp + 2 > kWaveStack) {
         if (b0 | b1) *err = 1;
         return;
@@ -150,9 +152,8 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
     const int e0 = child_entry(ni, 0, r0, c0), e1 = child_entry(ni, 1, r1, c1);
     if (b0 && b1) {
         // near child (popped first) = the one nearer for most active lanes
-        const uint64_t pref0 = __builtin_amdgcn_ballot_w64(active & (m0 <= m1));
-        const uint64_t act = __builtin_amdgcn_ballot_w64(active);
-        const bool near0 = 2 * __popcll(pref0) >= __popcll(act);
+        const uint64_t pref0 = __builtin_amdgcn_ballot_w64(m0 <= m1) & amask;
+        const bool near0 = 2 * __popcll(pref0) >= __popcll(amask);
         wstk[sp++] = near0 ? e1 : e0;
         wstk[sp++] = near0 ? e0 : e1;
     } else if (b0) {
@@ -201,8 +202,9 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
                            TopK<K>& tk, int* wstk, int* err) {
     CFloat* nf = (CFloat*)t.nodes;
     CFloat* pf = (CFloat*)t.pts;
+    const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
     int sp = 0;
-    packet_push_children<K>(nf, 0, qx, qy, qz, active, tk, wstk, sp, err);
+    packet_push_children<K>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err);
     while (sp > 0) {
         --sp;
         const int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
@@ -212,10 +214,10 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
             CInt* fi = (CInt*)f;
             CFloat* bx = f + 6 * s;
             const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
-            if (__builtin_amdgcn_ballot_w64(active & (md <= tk.worst_d2())) == 0) continue;
+            if ((__builtin_amdgcn_ballot_w64(md <= tk.worst_d2()) & amask) == 0) continue;
             packet_scan_leaf<K, NODUP>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk);
         } else {
-            packet_push_children<K>(nf, e >> 1, qx, qy, qz, active, tk, wstk, sp, err);
+            packet_push_children<K>(nf, e >> 1, qx, qy, qz, amask, tk, wstk, sp, err);
         }
     }
 }
@@ -231,8 +233,9 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
                                   TopK<K>& tk, int* wstk, int* lq, int* err) {
     CFloat* nf = (CFloat*)t.nodes;
     CFloat* pf = (CFloat*)t.pts;
+    const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
     int sp = 0;
-    packet_push_children<K>(nf, 0, qx, qy, qz, active, tk, wstk, sp, err);
+    packet_push_children<K>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err);
     while (sp > 0) {
         int nq = 0;
         while (sp > 0 && nq < kLeafQueue) {
@@ -241,7 +244,7 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
             if (e & 1)
                 lq[nq++] = e;
             else
-                packet_push_children<K>(nf, e >> 1, qx, qy, qz, active, tk, wstk, sp, err);
+                packet_push_children<K>(nf, e >> 1, qx, qy, qz, amask, tk, wstk, sp, err);
         }
         for (int i = 0; i < nq; ++i) {
             const int e = __builtin_amdgcn_readfirstlane(lq[i]);
@@ -250,7 +253,7 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
             CInt* fi = (CInt*)f;
             CFloat* bx = f + 6 * s;
             const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
-            if (__builtin_amdgcn_ballot_w64(active & (md <= tk.worst_d2())) == 0) continue;
+            if ((__builtin_amdgcn_ballot_w64(md <= tk.worst_d2()) & amask) == 0) continue;
             packet_scan_leaf<K, NODUP>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk);
         }
     }
