@@ -179,6 +179,7 @@ def main():
                                                    (avg_kernel_ms * 1e-3) / 1e9, 1)
                                              if "TRAFFIC_BYTES_PER_LAUNCH" in pmc else None),
                         "l2_hit_rate": pmc.get("L2_HIT_RATE"),
+                        "valu_busy": pmc.get("VALU_BUSY"),
                         "kernel_ms": round(avg_kernel_ms, 3)}
             rb = alg["radii_bytes_total"]
             rach = rb / (radii_ms * 1e-3) / 1e9
@@ -230,10 +231,17 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def kernel_source_sha256():
+    import hashlib
+    src = os.path.join(ROOT, "renderer_ray_marching_amd", "csrc", "rm_kernels.hip")
+    return hashlib.sha256(open(src, "rb").read()).hexdigest()
+
+
 def measured_traffic(workload, tune):
     """HBM bytes per launch of the march kernel from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE passes, calibrated with a 1 GiB copy), used only when that
-    summary was taken on the same workload with the same launch tune."""
+    summary was taken on the same workload, with the same launch tune and from the same kernel
+    source (SHA-256 of rm_kernels.hip)."""
     path = os.path.join(ROOT, "profiles", "march_batch_room10m_pmc.txt")
     if not os.path.exists(path):
         return {}
@@ -245,8 +253,13 @@ def measured_traffic(workload, tune):
     try:
         if rec.get("WORKLOAD") != workload or json.loads(rec.get("TUNE", "null")) != tune:
             return {}
-        return {"TRAFFIC_BYTES_PER_LAUNCH": int(rec["TRAFFIC_BYTES_PER_LAUNCH"]),
-                "L2_HIT_RATE": float(rec["L2_HIT_RATE"])}
+        if rec.get("KERNEL_SRC_SHA256") != kernel_source_sha256():
+            return {}
+        out = {"TRAFFIC_BYTES_PER_LAUNCH": int(rec["TRAFFIC_BYTES_PER_LAUNCH"]),
+               "L2_HIT_RATE": float(rec["L2_HIT_RATE"])}
+        if "VALU_BUSY" in rec:
+            out["VALU_BUSY"] = float(rec["VALU_BUSY"])
+        return out
     except (KeyError, ValueError):
         return {}
 
