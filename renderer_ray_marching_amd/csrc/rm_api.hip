@@ -92,6 +92,7 @@ DevTree dev_tree(const rm_tree* t) {
     d.pts = t->pts;
     d.n = t->n;
     d.depth = t->depth;
+    d.leaf_max = t->leaf_max;
     return d;
 }
 

@@ -32,6 +32,7 @@ struct DevTree {
     const float4* pts;      // leaf order: x, y, z, original index bits
     int64_t n;
     int depth;
+    int leaf_max;           // every leaf holds leaf_max points except the last one
 };
 
 struct DevCloud {

@@ -197,9 +197,11 @@ __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, flo
 // Exact top-K of every active lane's query. Must be called by the whole wavefront
 // (convergent); inactive lanes must hold keys that reject every insert (init(0)). NODUP: the
 // list was seeded with real points, so a candidate already listed is skipped.
+// Leaves whose points lie in [skip_b, skip_e) of the leaf order are not scanned (the caller has
+// scored those points already).
 template <int K, bool NODUP = false>
 __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool active,
-                           TopK<K>& tk, int* wstk, int* err) {
+                           TopK<K>& tk, int* wstk, int* err, int skip_b = 0, int skip_e = 0) {
     CFloat* nf = (CFloat*)t.nodes;
     CFloat* pf = (CFloat*)t.pts;
     const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
@@ -215,7 +217,9 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
             CFloat* bx = f + 6 * s;
             const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
             if ((__builtin_amdgcn_ballot_w64(md <= tk.worst_d2()) & amask) == 0) continue;
-            packet_scan_leaf<K, NODUP>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk);
+            const int b = fi[12 + s];
+            if (b >= skip_b && b < skip_e) continue;
+            packet_scan_leaf<K, NODUP>(pf, b, fi[14 + s], qx, qy, qz, tk);
         } else {
             packet_push_children<K>(nf, e >> 1, qx, qy, qz, amask, tk, wstk, sp, err);
         }
@@ -404,12 +408,19 @@ __global__ __launch_bounds__(WPB * 64) void radii_kernel(DevTree t, float max_ra
                                                          float* __restrict__ r_cl, int* err) {
     __shared__ int wstk[WPB][kWaveStack];
     int w = threadIdx.x >> 6;
-    int64_t pos = ((int64_t)blockIdx.x * WPB + w) * 64 + lane_id();
+    const int64_t g0 = ((int64_t)blockIdx.x * WPB + w) * 64;   // first point of the group
+    int64_t pos = g0 + lane_id();
     bool active = pos < t.n;
     float4 q = active ? t.pts[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
     TopK<2> tk;
     tk.init(active ? kPadKey : 0ull);
-    knn_packet<2>(t, q.x, q.y, q.z, active, tk, wstk[w], err);
+    // Every lane first scores the wavefront's own 64 points, which gives each lane a tight
+    // bound before the walk from the root; the walk then skips those leaves. The group is whole
+    // leaves when leaf_max divides 64 (all leaves hold leaf_max points except the last one);
+    // otherwise there is no pre-scan.
+    const int gcnt = (64 % t.leaf_max == 0) ? (int)min((int64_t)64, t.n - g0) : 0;
+    packet_scan_leaf<2, false>((CFloat*)t.pts, (int)g0, gcnt, q.x, q.y, q.z, tk);
+    knn_packet<2>(t, q.x, q.y, q.z, active, tk, wstk[w], err, (int)g0, (int)(g0 + gcnt));
     if (active) {
         float r = sqrtf(key_d2(tk.key[1]));
         uint32_t o = __float_as_uint(q.w);

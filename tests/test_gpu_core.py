@@ -156,6 +156,19 @@ def test_radii_bit_exact_box_100k(rm_ctx, leaf):
     assert (rc.cpu().numpy() == orc).all()
 
 
+@pytest.mark.parametrize("leaf,n", [(1, 3001), (3, 5003), (8, 4099), (16, 7000), (24, 6001),
+                                    (32, 70), (64, 50)])
+def test_radii_leaf_sizes_and_ragged_tails(rm_ctx, leaf, n):
+    # leaf sizes that do and do not divide the 64-point wavefront group, tails of every length
+    c = scenes.box_scene(n, seed=leaf + 3)
+    p = c.xyzw()
+    t = _tree(rm_ctx, p, leaf)
+    ru, rc = t.radii(0.05)
+    rm_ctx.sync()
+    oru, orc, _ = ob.RefTree(p, 64).radii(0.05)
+    assert (ru.cpu().numpy() == oru).all() and (rc.cpu().numpy() == orc).all()
+
+
 def test_radii_room_2m_full_size(rm_ctx):
     c = scenes.room_scene(2_000_000, seed=2)
     p = c.xyzw()
