@@ -34,7 +34,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="room10m", choices=["room10m", "room2m", "box100k"])
+    ap.add_argument("--workload", default="room10m",
+                    choices=["room10m", "room2m", "box100k", "scale50m"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch tune")
     ap.add_argument("--tune", default=None,
@@ -60,7 +61,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     wl = {"room10m": scenes.workload_room_10m, "room2m": scenes.workload_room_2m,
-          "box100k": scenes.workload_box_100k}[a.workload]
+          "box100k": scenes.workload_box_100k, "scale50m": scenes.workload_scale_50m}[a.workload]
     t0 = time.time()
     cloud, views, max_r = wl()
     log(f"scene {a.workload}: {cloud.n} points, {len(views)} views ({time.time() - t0:.1f}s)")

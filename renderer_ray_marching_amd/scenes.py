@@ -321,3 +321,17 @@ def workload_room_10m(n_points: int = 10_000_000):
     sizes = [sizes[i] for i in order]
     views = room_views(len(sizes), spec, seed=103, sizes=sizes)
     return cloud, views, 0.05
+
+
+def scale_50m_spec() -> RoomSpec:
+    return RoomSpec(size=(40.0, 30.0, 4.0), n_boxes=3000, outlier_frac=0.01)
+
+
+def workload_scale_50m(n_points: int = 50_000_000, n_views: int = 1024):
+    """50M points in a 40x30x4 m room (3000 boxes, 1 % outliers, radii clamped at 0.05 m);
+    1024 views at 1920x1080, fx = fy = 803.8 (2.12G rays)."""
+    spec = scale_50m_spec()
+    cloud = room_scene(n_points, seed=5, spec=spec)
+    views = room_views(n_views, spec, seed=105, sizes=((1920, 1080, 803.8),))
+    return cloud, views, 0.05
+

@@ -189,3 +189,68 @@ def test_adam_refinement_reduces_loss(rm_ctx):
     _, _, _, losses = diff.refine(R, x0, r0, c0, timg, tdep, steps=40)
     assert np.isfinite(losses).all()
     assert losses[-1] < 0.7 * losses[0], losses[::8]
+
+
+def test_central_fd_1000_points_full_workload(rm_ctx):
+    """The differentiable workload (1M box-scene points, 16 views at 512x512, fx 450): central
+    differences with step 1e-4 m on 1000 random non-silhouette points, for every centre
+    coordinate and the radius. Each pixel's depth depends only on its hit point, so one
+    perturbed render of the whole cloud gives every point's own finite difference: its loss
+    is summed over the pixels it hits, and a point is used only where all of its pixels keep
+    their hit in both perturbed renders and sit well inside their sphere's disc."""
+    diff = _diff()
+    cloud, views, max_r = scenes.workload_box_1m_diff()
+    rt = ob.RefTree(cloud.xyzw(), 64)
+    _, rc, _ = rt.radii(max_r)
+    xyz = torch.from_numpy(cloud.xyz).cuda()
+    r = torch.from_numpy(rc).cuda()
+    rgb = torch.from_numpy(cloud.colors4()[:, :3].copy()).cuda()
+    R = diff.DiffRenderer(xyz, r, rgb, views, ctx=rm_ctx)
+    rng = np.random.default_rng(7)
+    img, depth, hit = R.render(xyz, r, rgb)
+    V, H, W = hit.shape
+    hv = hit.cpu().numpy().reshape(V, -1)
+    tgt_d = depth + torch.from_numpy(rng.uniform(0.01, 0.05, depth.shape).astype(np.float32)).cuda()
+    xyz64, r64 = cloud.xyz.astype(np.float64), rc.astype(np.float64)
+    mask = np.zeros(hv.shape, bool)
+    bad = np.zeros(cloud.n, bool)
+    for v, view in enumerate(views):
+        lin = np.nonzero(hv[v] >= 0)[0]
+        q = _disc_ok(view, xyz64, r64, hv[v][lin], lin)
+        mask[v, lin[q > 0.01]] = True
+        bad[hv[v][lin][q < 0.2]] = True
+    m = torch.from_numpy(mask.reshape(V, H, W).astype(np.float32)).cuda()
+    x = xyz.clone().requires_grad_(True)
+    rr = r.clone().requires_grad_(True)
+    _, d, _ = R.render(x, rr, rgb)
+    ((d - tgt_d).abs() * m).sum().backward()
+    g = np.concatenate([x.grad.cpu().numpy(), rr.grad.cpu().numpy()[:, None]], 1)
+    pix_pt = hv[mask]                                # hit point of every masked pixel
+    npix = np.bincount(pix_pt, minlength=cloud.n)
+    step = 1e-4
+    ok = (npix >= 4) & ~bad
+    fd = np.zeros((cloud.n, 4))
+    for a in range(4):
+        lo = []
+        for sgn in (1.0, -1.0):
+            xp, rp = xyz.clone(), r.clone()
+            if a < 3:
+                xp[:, a] += sgn * step
+            else:
+                rp += sgn * step
+            _, dp, hp = R.render(xp, rp, rgb)
+            hp = hp.cpu().numpy().reshape(V, -1)
+            e = ((dp.double() - tgt_d.double()).abs()).cpu().numpy().reshape(V, -1)[mask]
+            lo.append(np.bincount(pix_pt, weights=e, minlength=cloud.n))
+            # a point whose pixels change hit in either render is not comparable
+            changed = np.bincount(pix_pt, weights=(hp[mask] != pix_pt), minlength=cloud.n)
+            ok &= changed == 0
+        fd[:, a] = (lo[0] - lo[1]) / (2 * step)
+    R.render(xyz, r, rgb)          # restore the tree to the unperturbed centres
+    cand = np.nonzero(ok)[0]
+    assert cand.size >= 1000, cand.size
+    pick = rng.choice(cand, 1000, replace=False)
+    err = np.abs(fd[pick] - g[pick])
+    tol = 0.02 * np.maximum(1.0, np.abs(g[pick]))
+    assert (err <= tol).mean() > 0.995, (err / tol).max()
+    assert np.median(err / np.maximum(1.0, np.abs(g[pick]))) < 2e-3
