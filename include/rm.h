@@ -162,6 +162,13 @@ int rm_render_batch(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
 int rm_render_diff(rm_ctx* ctx, const rm_cloud* cloud, const float* pose, float fov_radians,
                    int W, int H, float* rgb, float* depth, int32_t* hit);
 
+/* Forward of the differentiable form for every view of a batch (no counterpart in the
+ * reference): one launch of the batch's march (its rgba / hit buffers receive the marched
+ * output; hit must have been given at creation), then rgb (total pixels x 3) and depth (total
+ * pixels) as rm_render_diff defines them, at each view's offset. Equal to rm_render_diff view
+ * by view. */
+int rm_batch_render_diff(rm_batch* batch, float* rgb, float* depth);
+
 /* Adjoint of rm_render_diff with the hit map held fixed. grad_rgb: H x W x 3, grad_depth:
  * H x W upstream gradients. Accumulates (adds) into g_xyz (n x 3), g_r (n), g_rgb (n x 3) in
  * the original point order. Per-point sums are formed by sorting the hit pixels by point

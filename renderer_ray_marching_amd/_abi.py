@@ -68,6 +68,7 @@ SIGNATURES = {
     "rm_batch_autotune": (_i32, [_vp, _i32, C.POINTER(rm_tune)]),
     "rm_batch_get_tune": (_i32, [_vp, C.POINTER(rm_tune)]),
     "rm_batch_launch": (_i32, [_vp]),
+    "rm_batch_render_diff": (_i32, [_vp, _vp, _vp]),
     "rm_batch_destroy": (_i32, [_vp]),
     "rm_render_batch": (_i32, [_vp, _vp, C.POINTER(rm_view), _i32, _vp, _vp]),
     "rm_render_diff": (_i32, [_vp, _vp, C.POINTER(_f32), _f32, _i32, _i32, _vp, _vp, _vp]),

@@ -702,6 +702,27 @@ int rm_batch_launch(rm_batch* b) {
     return RM_OK;
 }
 
+int rm_batch_render_diff(rm_batch* b, float* rgb, float* depth) {
+    if (!b) return RM_ERR_INVALID;
+    rm_ctx* ctx = b->ctx;
+    if (!rgb || !depth) return fail(ctx, RM_ERR_INVALID, "rm_batch_render_diff: null output");
+    if (!b->hit)
+        return fail(ctx, RM_ERR_INVALID, "rm_batch_render_diff: the batch has no hit buffer");
+    if (int r = set_device(ctx)) return r;
+    DevCloud dc = dev_cloud(b->cloud);
+    timing_begin(ctx);
+    RM_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    for (const rm_view& v : b->views) {
+        const DevView dv = dev_view(v.pose, v.fov_radians, v.width, v.height, 0, 0, 0);
+        RM_HIP(ctx, rm::launch_diff_forward(ctx->stream, dc, dv,
+                                            reinterpret_cast<const float4*>(b->rgba) + v.offset,
+                                            b->hit + v.offset, rgb + 3 * v.offset,
+                                            depth + v.offset));
+    }
+    timing_end(ctx);
+    return RM_OK;
+}
+
 int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
     if (!b) return RM_ERR_INVALID;
     rm_ctx* ctx = b->ctx;

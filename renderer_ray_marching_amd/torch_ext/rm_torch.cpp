@@ -1,11 +1,13 @@
 // libtorch front end of the differentiable march: a torch::autograd::Function whose forward
-// renders every view with rm_render_diff (hit selection by the march, analytic sphere depth,
-// flat point colour) and whose backward runs the adjoint kernels (rm_render_adjoint), which
-// reduce per-point gradients by sorting the hit pixels by point index. All arithmetic runs in
-// librm_hip.so; this file only moves tensors across the C ABI.
+// renders all views in one batched march (rm_batch_render_diff: hit selection by the march,
+// analytic sphere depth, flat point colour) and whose backward runs the adjoint kernels
+// (rm_render_adjoint per view), which reduce per-point gradients by sorting the hit pixels by
+// point index. All arithmetic runs in librm_hip.so; this file only moves tensors across the
+// C ABI.
 #include <torch/script.h>
 #include <torch/torch.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -24,17 +26,28 @@ struct ViewSpec {
 };
 
 // Scene state shared by forward and backward: the library context, the tree built over the
-// initial points (refitted to every new set of centres) and the cloud bound to it.
+// initial points (refitted to every new set of centres), the cloud bound to it, and a batch over
+// all views (created at the first forward; its graph captures the march's output buffers, which
+// the scene therefore owns).
 struct DiffScene : torch::CustomClassHolder {
     rm_ctx* ctx = nullptr;
     rm_cloud* cloud = nullptr;
     int64_t n = 0;
     std::vector<ViewSpec> views;
+    rm_batch* batch = nullptr;
+    torch::Tensor rgba_buf, hit_buf;
 
     DiffScene(int64_t ctx_ptr, int64_t cloud_ptr, int64_t n_points)
         : ctx(reinterpret_cast<rm_ctx*>(ctx_ptr)),
           cloud(reinterpret_cast<rm_cloud*>(cloud_ptr)),
           n(n_points) {}
+
+    ~DiffScene() override { drop_batch(); }
+
+    void drop_batch() {
+        if (batch) rm_batch_destroy(batch);
+        batch = nullptr;
+    }
 
     void add_view(std::vector<double> pose, double fov, int64_t W, int64_t H) {
         TORCH_CHECK(pose.size() == 16, "pose must have 16 entries");
@@ -44,6 +57,27 @@ struct DiffScene : torch::CustomClassHolder {
         v.W = (int)W;
         v.H = (int)H;
         views.push_back(v);
+        drop_batch();
+    }
+
+    void ensure_batch(const torch::TensorOptions& fopt) {
+        if (batch) return;
+        const int64_t V = (int64_t)views.size();
+        const int64_t W = views[0].W, H = views[0].H;
+        rgba_buf = torch::empty({V * H * W * 4}, fopt);
+        hit_buf = torch::empty({V, H, W}, fopt.dtype(torch::kInt32));
+        std::vector<rm_view> rv((size_t)V);
+        for (int64_t v = 0; v < V; ++v) {
+            std::copy(views[v].pose.begin(), views[v].pose.end(), rv[v].pose);
+            rv[v].fov_radians = views[v].fov;
+            rv[v].width = (int32_t)W;
+            rv[v].height = (int32_t)H;
+            rv[v].reserved = 0;
+            rv[v].offset = v * H * W;
+        }
+        check_rm(rm_batch_create(ctx, cloud, rv.data(), (int)V, nullptr,
+                                 rgba_buf.data_ptr<float>(), hit_buf.data_ptr<int32_t>(), &batch),
+                 ctx, "rm_batch_create");
     }
 };
 
@@ -81,14 +115,11 @@ class RenderFunction : public torch::autograd::Function<RenderFunction> {
         auto fopt = xyz.options();
         auto img = torch::empty({V, H, W, 3}, fopt);
         auto depth = torch::empty({V, H, W}, fopt);
-        auto hit = torch::empty({V, H, W}, fopt.dtype(torch::kInt32));
-        for (int64_t v = 0; v < V; ++v) {
-            auto& vs = s.views[v];
-            check_rm(rm_render_diff(s.ctx, s.cloud, vs.pose.data(), vs.fov, W, H,
-                                    img[v].data_ptr<float>(), depth[v].data_ptr<float>(),
-                                    hit[v].data_ptr<int32_t>()),
-                     s.ctx, "rm_render_diff");
-        }
+        s.ensure_batch(fopt);
+        check_rm(rm_batch_render_diff(s.batch, img.data_ptr<float>(), depth.data_ptr<float>()),
+                 s.ctx, "rm_batch_render_diff");
+        // the batch reuses its hit buffer at the next forward; backward needs this one
+        auto hit = s.hit_buf.clone();
         actx->saved_data["scene"] = scene;
         actx->save_for_backward({hit});
         actx->mark_non_differentiable({hit});

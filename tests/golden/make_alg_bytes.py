@@ -9,7 +9,7 @@ A = far-child box tests (24 B), 10 (4 + 4) = the index remap and radius gathers 
 neighbours per step, 16 + 4 = the RGBA32F and hit-index writes. The march is counted on the
 pixels with (x + 7 y) mod 64 == 0 of every view and scaled by the exact pixel ratio.
 
-    python tests/golden/make_alg_bytes.py [--workload room10m|box100k|room2m]
+    python tests/golden/make_alg_bytes.py [--workload room10m|box100k|room2m|scale50m]
 """
 import argparse
 import json
@@ -31,6 +31,7 @@ WORKLOADS = {
     "box100k": scenes.workload_box_100k,
     "room2m": scenes.workload_room_2m,
     "room10m": scenes.workload_room_10m,
+    "scale50m": scenes.workload_scale_50m,
 }
 
 

@@ -254,3 +254,25 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
     tol = 0.02 * np.maximum(1.0, np.abs(g[pick]))
     assert (err <= tol).mean() > 0.995, (err / tol).max()
     assert np.median(err / np.maximum(1.0, np.abs(g[pick]))) < 2e-3
+
+
+def test_batched_forward_equals_per_view(rm_ctx, small_scene):
+    """The op's forward (one batched march over all views, rm_batch_render_diff) equals
+    rm_render_diff called view by view, on every pixel."""
+    import ctypes as C
+    from renderer_ray_marching_amd import _abi
+    cloud, rc, views, R, xyz, r, rgb = small_scene
+    img, depth, hit = R.render(xyz, r, rgb)
+    L = _abi.lib()
+    for v, view in enumerate(views):
+        W, H = view.width, view.height
+        im1 = torch.empty((H, W, 3), device="cuda")
+        d1 = torch.empty((H, W), device="cuda")
+        h1 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        pose = (C.c_float * 16)(*[float(x) for x in view.pose_colmajor()])
+        rc_ = L.rm_render_diff(rm_ctx.h, R.marcher.h, pose, C.c_float(view.fov), W, H,
+                               C.c_void_p(im1.data_ptr()), C.c_void_p(d1.data_ptr()),
+                               C.c_void_p(h1.data_ptr()))
+        assert rc_ == 0
+        rm_ctx.sync()
+        assert torch.equal(h1, hit[v]) and torch.equal(im1, img[v]) and torch.equal(d1, depth[v])
