@@ -115,6 +115,17 @@ __device__ __forceinline__ void sort10(uint64_t* k) {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Workgroups are dispatched to the chip's 8 XCDs round-robin (block b runs on XCD b mod 8), and
+// each XCD has its own L2. This renumbers blocks so that every XCD works on one contiguous run of
+// the logical block range: neighbouring work shares an L2. A bijection on [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int xcd = b % kXcds, local = b / kXcds;
+    const int base = nb / kXcds, extra = nb % kXcds;
+    return xcd * base + min(xcd, extra) + local;
+}
+
 // ------------------------------------------------------------------------------------------
 // Stack entries (both traversals): an inner child is pushed as (node << 1), a leaf child as
 // (parent << 2 | slot << 1 | 1). A leaf's box is re-tested against the current k-th distance
@@ -408,7 +419,7 @@ __global__ __launch_bounds__(WPB * 64) void radii_kernel(DevTree t, float max_ra
                                                          float* __restrict__ r_cl, int* err) {
     __shared__ int wstk[WPB][kWaveStack];
     int w = threadIdx.x >> 6;
-    const int64_t g0 = ((int64_t)blockIdx.x * WPB + w) * 64;   // first point of the group
+    const int64_t g0 = ((int64_t)xcd_block() * WPB + w) * 64;  // first point of the group
     int64_t pos = g0 + lane_id();
     bool active = pos < t.n;
     float4 q = active ? t.pts[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
