@@ -188,7 +188,9 @@ def main():
                           "frac": round(rach / HBM_PEAK_GBS, 4), "kernel_ms": round(radii_ms, 4),
                           "alg_bytes_per_query": round(alg["radii_bytes_per_query"], 1),
                           "compulsory_frac": round(16 * cloud.n / (radii_ms * 1e-3) / 1e9 /
-                                                   HBM_PEAK_GBS, 5)}
+                                                   HBM_PEAK_GBS, 5),
+                          "traffic": pmc.get("RADII_TRAFFIC_BYTES"),
+                          "l2_hit_rate": pmc.get("RADII_L2_HIT_RATE")}
 
     del batch, plain_out, marcher, tree, p, col
     torch.cuda.empty_cache()
@@ -197,7 +199,7 @@ def main():
     # ---- CPU baseline: the oracle's march on a bounded pixel sample, all host cores ----
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(cloud, views, rc.cpu().numpy())
+        cpu = cpu_baseline(cloud, views, rc.cpu().numpy(), max_r)
 
     out = {
         "metric": "Mrays/s batched vs per-view, 10M pts, >=99.9% hit match; fwd+bwd ms/view",
@@ -260,6 +262,10 @@ def measured_traffic(workload, tune):
                "L2_HIT_RATE": float(rec["L2_HIT_RATE"])}
         if "VALU_BUSY" in rec:
             out["VALU_BUSY"] = float(rec["VALU_BUSY"])
+        if "RADII_TRAFFIC_BYTES" in rec:
+            out["RADII_TRAFFIC_BYTES"] = int(rec["RADII_TRAFFIC_BYTES"])
+        if "RADII_L2_HIT_RATE" in rec:
+            out["RADII_L2_HIT_RATE"] = float(rec["RADII_L2_HIT_RATE"])
         return out
     except (KeyError, ValueError):
         return {}
@@ -319,9 +325,21 @@ def diff_leg(api, scenes, torch, steps):
             "loss_last": round(losses[-1], 2)}
 
 
-def cpu_baseline(cloud, views, radii):
-    """Oracle march (C++/OpenMP restatement of the reference algorithm) over every 16th pixel
-    of every view, on all host cores available to this process."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cloud, views, radii, max_r):
+    """The oracle (C++/OpenMP restatement of the reference algorithm) on the host cores
+    available to this process: the march over every 16th pixel of every view (the headline
+    value), the same march on one thread over every 256th pixel of the first 16 views, and the
+    k = 2 radii over all points."""
     from oracle import binding as ob
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     ncores = max(1, min(ncores, 64))
@@ -329,21 +347,37 @@ def cpu_baseline(cloud, views, radii):
     tree = ob.RefTree(cloud.xyzw(), 64)
     log(f"cpu baseline: oracle tree ({time.time() - t0:.1f}s)")
     col = cloud.colors4()
+
+    def march(stride, vs, nthreads):
+        rays, tcpu = 0, 0.0
+        for v in vs:
+            pix = np.arange(0, v.width * v.height, stride, dtype=np.int32)
+            t1 = time.perf_counter()
+            tree.render(radii, col, v.pose_colmajor(), v.fov, v.width, v.height, pix=pix,
+                        nthreads=nthreads)
+            tcpu += time.perf_counter() - t1
+            rays += pix.shape[0]
+        return rays, tcpu
+
     stride = 16
-    rays = 0
-    tcpu = 0.0
-    for v in views:
-        pix = np.arange(0, v.width * v.height, stride, dtype=np.int32)
-        t1 = time.perf_counter()
-        tree.render(radii, col, v.pose_colmajor(), v.fov, v.width, v.height, pix=pix,
-                    nthreads=ncores)
-        tcpu += time.perf_counter() - t1
-        rays += pix.shape[0]
-    log(f"cpu baseline: {rays} rays in {tcpu:.1f}s")
+    rays, tcpu = march(stride, views, ncores)
+    log(f"cpu baseline: {rays} rays in {tcpu:.1f}s on {ncores} threads")
+    rays1, tcpu1 = march(256, views[:16], 1)
+    log(f"cpu baseline: {rays1} rays in {tcpu1:.1f}s on 1 thread")
+    t1 = time.perf_counter()
+    tree.radii(max_r, nthreads=ncores)
+    radii_s = time.perf_counter() - t1
+    log(f"cpu baseline: radii of {cloud.n} points in {radii_s:.2f}s")
     return {"value": round(rays / tcpu / 1e6, 4), "unit": "Mrays/s", "cores": ncores,
             "kind": "port",
             "sample": f"every {stride}th pixel of all {len(views)} views ({rays} rays), "
-                      "oracle restatement of the reference march, OpenMP"}
+                      "oracle restatement of the reference march, OpenMP",
+            "value_1thread": round(rays1 / tcpu1 / 1e6, 4),
+            "sample_1thread": f"every 256th pixel of the first {min(16, len(views))} views "
+                              f"({rays1} rays)",
+            "radii_s": round(radii_s, 3),
+            "radii_sample": f"k = 2 radii of all {cloud.n} points, {ncores} threads",
+            "cpu_model": cpu_model()}
 
 
 if __name__ == "__main__":
