@@ -48,6 +48,8 @@ def main():
                     help="Adam steps of the differentiable leg")
     ap.add_argument("--plain-views", type=int, default=0,
                     help="time the plain path on the first N views only (0 = all)")
+    ap.add_argument("--leaf", type=int, default=0,
+                    help="points per tree leaf (0 = the library default, 16)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -77,7 +79,7 @@ def main():
     p = torch.from_numpy(cloud.xyzw()).cuda()
     col = torch.from_numpy(cloud.colors4()).cuda()
     t0 = time.time()
-    tree = api.KDTreeFlann(ctx)
+    tree = api.KDTreeFlann(ctx, leaf_max=a.leaf)
     tree.Build(p)
     info = tree.info()
     log(f"tree: {info} ({time.time() - t0:.1f}s)")

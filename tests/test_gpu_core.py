@@ -343,3 +343,63 @@ def test_batch_autotune_keeps_results(rm_ctx, room_small):
     ref_rgba = torch.cat([p[0] for p in plain])
     assert torch.equal(b.hit, ref_hit) and torch.equal(b.rgba, ref_rgba)
     b.close()
+
+
+# ------------------------------------------------------------------------------------------
+# full-size configurations: the batched march equals the plain path on every pixel and the
+# oracle on a deterministic pixel subsample
+# ------------------------------------------------------------------------------------------
+
+def _subsample(W, H, step):
+    lin = np.arange(W * H, dtype=np.int64)
+    x, y = lin % W, lin // W
+    return lin[(x + 7 * y) % step == 0].astype(np.int32)
+
+
+def _full_size_check(rm_ctx, cloud, views, max_r, oracle_views, step):
+    p = cloud.xyzw()
+    t = _tree(rm_ctx, p)
+    ru, rc = t.radii(max_r)
+    m = api.PointcloudRayMarcher(_dev(p), _dev(cloud.colors4()), rc, t)
+    b = api.Batch(m, views)
+    b.launch()
+    rm_ctx.sync()
+    rt = ob.RefTree(p, 64)
+    oru, orc, _ = rt.radii(max_r)
+    assert (ru.cpu().numpy() == oru).all() and (rc.cpu().numpy() == orc).all()
+    offs = np.cumsum([0] + [v.width * v.height for v in views])
+    hit = b.hit.cpu().numpy()
+    rgba = b.rgba.cpu().numpy()
+    checked = 0
+    for i in oracle_views:
+        v = views[i]
+        pix = _subsample(v.width, v.height, step)
+        orgba, ohit, _ = rt.render(orc, cloud.colors4(), v.pose_colmajor(), v.fov, v.width,
+                                   v.height, pix=pix)
+        assert (hit[offs[i] + pix] == ohit).all(), i
+        assert (rgba[offs[i] + pix] == orgba).all(), i
+        checked += pix.size
+    # every pixel of a few views also against the plain per-view launch
+    for i in oracle_views[:2]:
+        v = views[i]
+        prgba, phit = m.render_to_texture(v.pose_colmajor(), v.fov, v.width, v.height)
+        rm_ctx.sync()
+        n = v.width * v.height
+        assert (phit.cpu().numpy().ravel() == hit[offs[i]:offs[i] + n]).all()
+        assert (prgba.cpu().numpy().reshape(-1, 4) == rgba[offs[i]:offs[i] + n]).all()
+    b.close()
+    return checked
+
+
+@pytest.mark.slow
+def test_full_size_room_2m_32_views(rm_ctx):
+    cloud, views, max_r = scenes.workload_room_2m()
+    n = _full_size_check(rm_ctx, cloud, views, max_r, list(range(0, 32, 4)), 16)
+    assert n > 300_000
+
+
+@pytest.mark.slow
+def test_full_size_room_10m_256_views(rm_ctx):
+    cloud, views, max_r = scenes.workload_room_10m()
+    n = _full_size_check(rm_ctx, cloud, views, max_r, [0, 37, 101, 190, 255], 16)
+    assert n > 100_000
