@@ -595,12 +595,13 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
 // wavefront stay screen neighbours and the tail of long rays is paid once per tile, not once per
 // 64 pixels. One wavefront step advances every live ray by one march step (one kNN-10 query
 // each), in packet (MODE 0) or per-lane (MODE 1) traversal.
-template <int MODE, bool WARM, int WPB>
 //
-// The kernel is latency-bound (scalar loads of nodes and points, LDS stack traffic between
-// dependent steps), so it is compiled for 8 wavefronts per SIMD (64 VGPRs): the few values that
-// do not fit are per-ray state spilled to scratch outside the traversal loops (measured 17 %
-// faster than 5 wavefronts per SIMD at 91 VGPRs without spills).
+// Occupancy: each traversal step waits on scalar loads of node records and points and on the
+// LDS stack, so the vector ALUs need many resident wavefronts to stay busy. The kernel is
+// compiled for 8 wavefronts per SIMD (64 VGPRs); the values that do not fit are per-ray state
+// spilled to scratch outside the traversal loops (measured 17 % faster than 5 wavefronts per SIMD
+// at 91 VGPRs without spills; the vector ALUs then issue 90 % of the time).
+template <int MODE, bool WARM, int WPB>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) void batch_kernel(
     BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
