@@ -66,15 +66,18 @@ int rm_tree_info(const rm_tree* tree, int64_t* n, int64_t* n_nodes, int* depth, 
  * fewer than k points. Replaces KDTreeFlann::Search / SearchKNN
  * (src/kdtree/kdtree_flann.cu:5-28, 65-80) and KDTreeCuda3dIndex::knnSearchGpu
  * (kdtree_cuda_3d_index.cu:261-399) including the index remap map_indices (:160-173).
- * Squared distance is ((dx*dx + dy*dy) + dz*dz) in float, no fused multiply-add. */
+ * Squared distance is fma(dz, dz, fma(dy, dy, dx*dx)) in float (each step rounded once). */
 int rm_knn(rm_ctx* ctx, const rm_tree* tree, const float* q_xyzw, int64_t m, int k,
            int32_t* idx, float* d2);
 
 /* Radius search: up to max_nn neighbours with d2 < radius^2 (strict, as the reference result
  * set KnnRadiusResultSet, util/cuda/result_set.h:372-476) per query, ascending by
  * (d2, index), padded with -1 / +INF; counts[i] = number found (<= max_nn, may be NULL).
+ * max_nn == 0 is the counting mode (CountingRadiusResultSet with no cap, result_set.h:225-267,
+ * kdtree_cuda_3d_index.cu:436-452): counts[i] = number of points with d2 < radius^2, uncapped;
+ * counts is required and idx, d2 are not touched (may be NULL).
  * Replaces KDTreeFlann::SearchRadius (src/kdtree/kdtree_flann.cu:82-99) and radiusSearchGpu
- * (kdtree_cuda_3d_index.cu:410-557). max_nn <= RM_MAX_K. */
+ * (kdtree_cuda_3d_index.cu:410-557). 0 <= max_nn <= RM_MAX_K. */
 int rm_radius_search(rm_ctx* ctx, const rm_tree* tree, const float* q_xyzw, int64_t m,
                      float radius, int max_nn, int32_t* idx, float* d2, int32_t* counts);
 

@@ -132,14 +132,21 @@ class KDTreeFlann:
         return knn * m, idx, d2
 
     def SearchRadius(self, query: torch.Tensor, radius: float, max_nn: int):
+        """Returns (total found, idx (M, max_nn), d2 (M, max_nn), per-query counts (M,)).
+
+        Neighbours have d2 < radius**2 (kdtree_flann.cu:82-99, result_set.h:372-476). With
+        max_nn == 0 only the uncapped per-query counts are computed (idx and d2 are (M, 0)),
+        the counting mode of radiusSearchGpu (kdtree_cuda_3d_index.cu:436-452), whose return
+        value is their sum."""
         q = query.reshape(-1, 4).contiguous()
         m = q.shape[0]
         idx = torch.empty((m, max_nn), dtype=torch.int32, device=q.device)
         d2 = torch.empty((m, max_nn), dtype=torch.float32, device=q.device)
         cnt = torch.empty((m,), dtype=torch.int32, device=q.device)
         check(lib().rm_radius_search(self.ctx.h, self.h, _ptr(q), m, float(radius), max_nn,
-                                     _ptr(idx), _ptr(d2), _ptr(cnt)), self.ctx.h)
-        return int(cnt.sum().item()), idx, d2, cnt
+                                     _ptr(idx) if max_nn else None, _ptr(d2) if max_nn else None,
+                                     _ptr(cnt)), self.ctx.h)
+        return int(cnt.sum(dtype=torch.int64).item()), idx, d2, cnt
 
     def radii(self, max_radius: float = 0.1):
         """Per-point radii of main.cu:99-149 (unclamped, clamped), original order."""

@@ -432,11 +432,12 @@ int rm_radius_search(rm_ctx* ctx, const rm_tree* tree, const float* q, int64_t m
                      int max_nn, int32_t* idx, float* d2, int32_t* counts) {
     if (!ctx) return RM_ERR_INVALID;
     if (int r = check_tree(ctx, tree)) return r;
-    if (max_nn < 1) return fail(ctx, RM_ERR_INVALID, "rm_radius_search: max_nn must be >= 1");
+    if (max_nn < 0) return fail(ctx, RM_ERR_INVALID, "rm_radius_search: max_nn must be >= 0");
     if (max_nn > RM_MAX_K)
         return fail(ctx, RM_ERR_UNSUPPORTED, "rm_radius_search: max_nn above RM_MAX_K");
     if (!(radius >= 0.0f)) return fail(ctx, RM_ERR_INVALID, "rm_radius_search: bad radius");
-    if (m < 0 || (m > 0 && (!q || !idx || !d2)))
+    // max_nn == 0 counts only (kdtree_cuda_3d_index.cu:436-452): counts required, idx/d2 unused
+    if (m < 0 || (m > 0 && (!q || (max_nn > 0 ? (!idx || !d2) : !counts))))
         return fail(ctx, RM_ERR_INVALID, "rm_radius_search: bad buffers");
     if (int r = set_device(ctx)) return r;
     float r2 = radius * radius;  // float(radius * radius), src/kdtree/kdtree_flann.cu:98

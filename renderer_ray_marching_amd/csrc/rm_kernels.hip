@@ -293,9 +293,9 @@ __device__ __forceinline__ NodeRec load_node(const RmNode* nodes, int ni) {
     return n;
 }
 
-template <int K>
+template <class Set>
 __device__ __forceinline__ void lane_push_children(const RmNode* nodes, int ni, float qx,
-                                                   float qy, float qz, const TopK<K>& tk,
+                                                   float qy, float qz, const Set& tk,
                                                    int* lstk, int stride, int& sp, int maxsp,
                                                    int* err) {
     NodeRec n = load_node(nodes, ni);
@@ -320,15 +320,15 @@ __device__ __forceinline__ void lane_push_children(const RmNode* nodes, int ni, 
     }
 }
 
-template <int K, bool NODUP = false>
-__device__ void knn_lane(const DevTree& t, float qx, float qy, float qz, TopK<K>& tk, int* lstk,
+template <bool NODUP = false, class Set>
+__device__ void knn_lane(const DevTree& t, float qx, float qy, float qz, Set& tk, int* lstk,
                          int stride, int maxsp, int* err) {
     int sp = 0;
-    lane_push_children<K>(t.nodes, 0, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
+    lane_push_children(t.nodes, 0, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
     while (sp > 0) {
         const int e = lstk[(--sp) * stride];
         if (!(e & 1)) {
-            lane_push_children<K>(t.nodes, e >> 1, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
+            lane_push_children(t.nodes, e >> 1, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
             continue;
         }
         const int ni = e >> 2, s = (e >> 1) & 1;
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void knn_kernel(DevTree t, const float* __rest
     float4 qq = reinterpret_cast<const float4*>(q)[i];
     TopK<K> tk;
     tk.init(kPadKey);
-    knn_lane<K>(t, qq.x, qq.y, qq.z, tk, lstk + threadIdx.x, blockDim.x, maxsp, err);
+    knn_lane(t, qq.x, qq.y, qq.z, tk, lstk + threadIdx.x, blockDim.x, maxsp, err);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (j < k) {
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void radius_kernel(DevTree t, const float* __r
     const uint64_t pad = make_key(r2, 0u);
     TopK<K> tk;
     tk.init(pad);
-    knn_lane<K>(t, qq.x, qq.y, qq.z, tk, lstk + threadIdx.x, blockDim.x, maxsp, err);
+    knn_lane(t, qq.x, qq.y, qq.z, tk, lstk + threadIdx.x, blockDim.x, maxsp, err);
     int c = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -409,6 +409,32 @@ __global__ __launch_bounds__(256) void radius_kernel(DevTree t, const float* __r
         }
     }
     if (counts) counts[i] = c;
+}
+
+// Counting result set (CountingRadiusResultSet with max_neighbors -1, util/cuda/result_set.h:
+// 225-267): the pruning bound stays at r2 and every candidate with d2 < r2 is counted, with no
+// cap. Takes the place of TopK in knn_lane.
+struct CountSet {
+    float r2;
+    int count;
+    __device__ __forceinline__ float worst_d2() const { return r2; }
+    template <bool NODUP>
+    __device__ __forceinline__ void add(uint64_t kk) {
+        count += key_d2(kk) < r2 ? 1 : 0;
+    }
+};
+
+__global__ __launch_bounds__(256) void radius_count_kernel(DevTree t, const float* __restrict__ q,
+                                                           int64_t m, float r2,
+                                                           int32_t* __restrict__ counts, int maxsp,
+                                                           int* err) {
+    extern __shared__ int lstk[];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    float4 qq = reinterpret_cast<const float4*>(q)[i];
+    CountSet cs{r2, 0};
+    knn_lane(t, qq.x, qq.y, qq.z, cs, lstk + threadIdx.x, blockDim.x, maxsp, err);
+    counts[i] = cs.count;
 }
 
 // Radii: one wavefront per 64 consecutive leaf-ordered points (spatially coherent), packet
@@ -543,10 +569,10 @@ __device__ void march_lane(const DevCloud& c, const Ray& r, int* lstk, int strid
         float pz = r.oz + r.dz * t;
         if (WARM && i > 0) {
             warm_seed(c, tk, px, py, pz);
-            knn_lane<kMarchK, true>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
+            knn_lane<true>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
         } else {
             tk.init(kPadKey);
-            knn_lane<kMarchK>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
+            knn_lane(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
         }
         float s;
         uint32_t id;
@@ -693,10 +719,10 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                                                  p.err);
             } else if (has) {
                 if (warm_lane)
-                    knn_lane<kMarchK, true>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
+                    knn_lane<true>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
                                             WPB * 64, maxsp, p.err);
                 else
-                    knn_lane<kMarchK>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
+                    knn_lane(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
                                       WPB * 64, maxsp, p.err);
             }
             if (has) {
@@ -1080,6 +1106,11 @@ hipError_t launch_radius(hipStream_t s, const DevTree& t, const float* q, int64_
     if (m == 0) return hipSuccess;
     int maxsp = lane_stack_depth(t);
     size_t shmem = (size_t)maxsp * 256 * sizeof(int);
+    if (max_nn == 0) {
+        hipLaunchKernelGGL(radius_count_kernel, dim3(grid_for(m, 256)), dim3(256), shmem, s, t, q,
+                           m, r2, counts, maxsp, err);
+        return hipGetLastError();
+    }
     int kk = max_nn <= 1 ? 1 : max_nn <= 2 ? 2 : max_nn <= 4 ? 4 : max_nn <= 8 ? 8
              : max_nn <= 10 ? 10 : max_nn <= 16 ? 16 : 32;
     switch (kk) {

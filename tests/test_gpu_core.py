@@ -1,12 +1,14 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle on the same seeded inputs.
 
 Integer/index outputs must be bit-exact; float outputs (d2, radii, depth, colour) are compared
-exactly too, since both sides evaluate the same float expressions without contraction.
+exactly too, since both sides evaluate the same float expressions: the squared distance as
+fma(dz, dz, fma(dy, dy, dx * dx)), everything else without contraction.
 """
 import numpy as np
 import pytest
 import torch
 
+import fp32
 from oracle import binding as ob
 from renderer_ray_marching_amd import api, scenes
 
@@ -138,6 +140,31 @@ def test_radius_search_vs_brute(rm_ctx):
     assert (idx.cpu().numpy() == exp_i).all()
     assert (d2.cpu().numpy() == exp_d).all()
     assert (cnt.cpu().numpy() == inside.sum(1)).all()
+
+
+@pytest.mark.parametrize("radius", [0.0, 0.05, 0.25])
+def test_radius_count_mode_vs_brute(rm_ctx, radius):
+    """max_nn == 0: uncapped per-query counts of d2 < r^2 (CountingRadiusResultSet with no cap),
+    checked by brute force over exact float32 fma distances; counts go well past RM_MAX_K."""
+    p = scenes.box_scene(20000, seed=5).xyzw()
+    q = _queries(p, 7, 900)
+    t = _tree(rm_ctx, p)
+    total, idx, d2, cnt = t.SearchRadius(_dev(q), radius, 0)
+    rm_ctx.sync()
+    assert idx.shape == (q.shape[0], 0) and d2.shape == (q.shape[0], 0)
+    r2 = np.float32(radius) * np.float32(radius)
+    exp = np.zeros(q.shape[0], np.int64)
+    for a in range(0, q.shape[0], 100):
+        exp[a:a + 100] = (fp32.dist2(p[None, :, :], q[a:a + 100, None, :]) < r2).sum(1)
+    got = cnt.cpu().numpy()
+    assert (got == exp).all()
+    assert total == int(exp.sum())
+    if radius == 0.25:
+        assert exp.max() > 32
+    # the capped mode agrees wherever the cap is not reached
+    _, _, _, cap = t.SearchRadius(_dev(q), radius, 32)
+    cap = cap.cpu().numpy()
+    assert (cap == np.minimum(exp, 32)).all()
 
 
 # ------------------------------------------------------------------------------------------
