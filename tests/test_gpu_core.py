@@ -10,7 +10,8 @@ import torch
 
 import fp32
 from oracle import binding as ob
-from renderer_ray_marching_amd import api, scenes
+from renderer_ray_marching_amd import _abi, api, scenes
+from renderer_ray_marching_amd._abi import RMError
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +90,29 @@ def test_knn_ties_duplicates_and_padding(rm_ctx):
     assert (idx[:, 5:] == -1).all() and np.isinf(d2[:, 5:]).all()
     oi, od = ob.knn_brute(small, q[:10], 8)
     assert (idx == oi).all() and (d2 == od).all()
+
+
+def test_empty_queries_and_empty_cloud(rm_ctx):
+    p = scenes.box_scene(2000, seed=3).xyzw()
+    t = _tree(rm_ctx, p)
+    q0 = torch.zeros((0, 4), dtype=torch.float32, device="cuda")
+    total, idx, d2 = t.SearchKNN(q0, 10)
+    assert total == 0 and idx.shape == (0, 10) and d2.shape == (0, 10)
+    total, idx, d2, cnt = t.SearchRadius(q0, 0.1, 8)
+    assert total == 0 and idx.shape == (0, 8) and cnt.shape == (0,)
+    total, _, _, cnt = t.SearchRadius(q0, 0.1, 0)
+    assert total == 0 and cnt.shape == (0,)
+    rm_ctx.sync()
+    with pytest.raises(RMError):
+        api.KDTreeFlann(rm_ctx).Build(q0)
+    q4, cnt = _dev(p[:4]), torch.zeros(4, dtype=torch.int32, device="cuda")
+    f = _abi.lib().rm_radius_search
+    assert f(rm_ctx.h, t.h, api._ptr(q4), 4, 0.1, -1, None, None, api._ptr(cnt)) == \
+        _abi.RM_ERR_INVALID                                   # negative max_nn
+    assert f(rm_ctx.h, t.h, api._ptr(q4), 4, 0.1, 0, None, None, None) == \
+        _abi.RM_ERR_INVALID                                   # counting mode without counts
+    assert f(rm_ctx.h, t.h, api._ptr(q4), 4, -1.0, 0, None, None, api._ptr(cnt)) == \
+        _abi.RM_ERR_INVALID                                   # negative radius
 
 
 @pytest.mark.parametrize("n,leaf", [(1, 16), (16, 16), (17, 16), (1000, 1), (50_000, 16),
