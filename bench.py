@@ -50,6 +50,8 @@ def main():
                     help="time the plain path on the first N views only (0 = all)")
     ap.add_argument("--leaf", type=int, default=0,
                     help="points per tree leaf (0 = the library default, 16)")
+    ap.add_argument("--cull", action="store_true",
+                    help="march over each view's frustum-culled cloud (off in the reference)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -96,6 +98,8 @@ def main():
     log(f"radii kernel: {radii_ms:.3f} ms")
 
     marcher = api.PointcloudRayMarcher.get_instance(p, col, rc, tree)
+    if a.cull:
+        marcher.set_culling(True)
 
     # ---- plain per-view path: one launch per view, sequential ----
     nplain = len(views) if a.plain_views <= 0 else min(a.plain_views, len(views))
@@ -159,6 +163,20 @@ def main():
     hit_agreement = agree / plain_rays
     hit_frac = float((batch.hit >= 0).float().mean().item())
     log(f"batched: {value:.1f} Mrays/s, agreement with plain {hit_agreement:.6f}")
+    cull_info = None
+    if a.cull:
+        # what culling changes: hit agreement with the unculled batched march
+        marcher.set_culling(False)
+        full = api.Batch(marcher, views, tune=tune)
+        full.launch()
+        ctx.sync()
+        cull_info = {"hit_agreement_vs_unculled":
+                     float((full.hit == batch.hit).float().mean().item()),
+                     "hit_fraction_unculled": round(float((full.hit >= 0).float().mean()
+                                                          .item()), 4)}
+        log(f"culled vs unculled hit agreement {cull_info['hit_agreement_vs_unculled']:.6f}")
+        full.close()
+        del full
 
     # ---- roofline (algorithmic bytes counted by the oracle, committed per workload) ----
     alg_path = os.path.join(ROOT, "tests", "golden", "alg_bytes.json")
@@ -167,7 +185,7 @@ def main():
     if os.path.exists(alg_path):
         with open(alg_path) as f:
             alg = json.load(f).get(a.workload)
-        if alg and alg["points"] == cloud.n and alg["rays"] == rays:
+        if alg and alg["points"] == cloud.n and alg["rays"] == rays and not a.cull:
             b_launch = alg["march_bytes_per_ray"] * rays
             ach = b_launch / (avg_kernel_ms * 1e-3) / 1e9
             pmc = measured_traffic(a.workload, tune)
@@ -233,6 +251,8 @@ def main():
         "diff": diff,
         "cpu_baseline": cpu,
     }
+    if a.cull:
+        out["cull"] = cull_info
     print(json.dumps(out), flush=True)
 
 

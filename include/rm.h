@@ -103,6 +103,15 @@ int rm_cloud_destroy(rm_cloud* cloud);
 int rm_cloud_set_radii(rm_ctx* ctx, rm_cloud* cloud, const float* radii);
 /* Replace the colours (n x 4 floats) of a cloud. */
 int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* cloud, const float* colors);
+/* Frustum culling (enable = 1) or none (0, the default, as the reference). With culling, each
+ * view marches over the points whose centres are inside its frustum: inside the four side
+ * planes through the camera origin and the image corner directions, beyond the near plane (0.1)
+ * and before the far plane (100) along the forward axis, i.e. the predicate is_inside of
+ * src/ray_marching.cu:231-258 with the edge points of :112-124, which the reference computes but
+ * has commented out. The kNN-10 of every march step is then exact over the inside points.
+ * Applies to rm_render, rm_render_diff and to batches created after the call (a batch keeps the
+ * setting it was created with). Enabling allocates one view's culling data on the cloud. */
+int rm_cloud_set_culling(rm_ctx* ctx, rm_cloud* cloud, int enable);
 /* Move the points of a cloud (xyz: n x 3 floats, original order) and refit the boxes of the
  * tree it was created from, keeping the topology. Search stays exact because every bound is
  * recomputed from the moved points. The tree is modified in place. */

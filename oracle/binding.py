@@ -64,6 +64,8 @@ def lib():
         L.orc_render_steps.argtypes = [C.c_void_p, _f, _f, _f, C.c_float, C.c_int, C.c_int,
                                        C.c_void_p, C.c_int64, _f, _i, _i,
                                        C.POINTER(Counters), C.c_int]
+        L.orc_frustum_inside.argtypes = [_f, C.c_float, C.c_int, C.c_int, _f, C.c_int64,
+                                         np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]
         L.orc_loss_grad_fixed.restype = C.c_double
         L.orc_loss_grad_fixed.argtypes = [_d, _d, _d, C.c_int64, _f, C.c_float, C.c_int,
                                           C.c_int, _i, _f, _f, C.c_double, C.c_double, _d, _d,
@@ -180,6 +182,32 @@ def knn_brute(pts, q, k, nthreads=0):
     d2 = np.empty((q.shape[0], k), np.float32)
     lib().orc_knn_brute(p, p.shape[0], q, q.shape[0], k, idx, d2, nthreads)
     return idx, d2
+
+
+def frustum_inside(pose_colmajor, fov, W, H, pts):
+    """1 where a point (n x 3 or n x 4) is inside the view frustum (src/ray_marching.cu:231-258)."""
+    p = _xyzw(pts)
+    out = np.empty(p.shape[0], np.uint8)
+    pose = np.ascontiguousarray(pose_colmajor, np.float32)
+    lib().orc_frustum_inside(pose, lib().orc_pane_distance(W, fov), W, H, p, p.shape[0], out)
+    return out
+
+
+def render_culled(pts, radii, colors4, pose_colmajor, fov, W, H, pix=None, nthreads=0):
+    """The march over the culled cloud: the points inside the view frustum (frustum_inside), in
+    their original order, with their radii and colours; hit indices refer to the full cloud.
+    Returns (rgba, hit, inside)."""
+    p = _xyzw(pts)
+    inside = frustum_inside(pose_colmajor, fov, W, H, p)
+    sub = np.nonzero(inside)[0].astype(np.int32)
+    npix = W * H if pix is None else len(pix)
+    if sub.size == 0:
+        return np.zeros((npix, 4), np.float32), np.full(npix, -1, np.int32), inside
+    t = RefTree(p[sub], 64)
+    rgba, hit, _ = t.render(np.asarray(radii, np.float32)[sub],
+                            np.asarray(colors4, np.float32)[sub], pose_colmajor, fov, W, H,
+                            pix=pix, nthreads=nthreads)
+    return rgba, np.where(hit >= 0, sub[np.maximum(hit, 0)], -1).astype(np.int32), inside
 
 
 def subsample_indices(n):

@@ -248,6 +248,58 @@ struct MarchHit {
     int32_t idx;
 };
 
+// The view frustum of render_to_texture (src/ray_marching.cu:112-124): the edge points
+// camera_matrix * v(x, y), v(x, y) = (x * (0.5 W), y * (0.5 H), -W / (2 tanf(0.5 fov)), 0), for
+// the corners (-1,-1), (1,-1), (1,1), (-1,1) in that order, with glm's mat4 * vec4 summed as
+// (m0 v0 + m1 v1) + (m2 v2 + m3 v3). The side planes of is_inside (:243-250) pair corner i with
+// corner i+1 and use the normal (v2 x v1) with v2 = corner i, v1 = corner i+1.
+struct Frustum {
+    float n[4][3];
+    float cam[3], dir[3];
+};
+
+Frustum make_frustum(const float* M, float pane, int W, int H) {
+    const float xs[4] = {-1.0f, 1.0f, 1.0f, -1.0f};
+    const float ys[4] = {-1.0f, -1.0f, 1.0f, 1.0f};
+    float e[4][3];
+    for (int k = 0; k < 4; ++k) {
+        float v[4] = {xs[k] * (0.5f * (float)W), ys[k] * (0.5f * (float)H), -pane, 0.0f};
+        for (int i = 0; i < 3; ++i) {
+            float m01 = M[0 + i] * v[0] + M[4 + i] * v[1];
+            float m23 = M[8 + i] * v[2] + M[12 + i] * v[3];
+            e[k][i] = m01 + m23;
+        }
+    }
+    Frustum f;
+    for (int k = 0; k < 4; ++k) {
+        const float* v2 = e[k];
+        const float* v1 = e[(k + 1) % 4];
+        f.n[k][0] = v2[1] * v1[2] - v2[2] * v1[1];
+        f.n[k][1] = v2[2] * v1[0] - v2[0] * v1[2];
+        f.n[k][2] = v2[0] * v1[1] - v2[1] * v1[0];
+    }
+    for (int i = 0; i < 3; ++i) {
+        f.cam[i] = M[12 + i];     // make_float3(CAMERA_MATRIX[3])
+        f.dir[i] = -M[8 + i];     // make_float3(-CAMERA_MATRIX[2])
+    }
+    return f;
+}
+
+// is_inside (src/ray_marching.cu:231-258, the commented-out body): four side planes, then the
+// near plane (pt - dir*ZNEAR) . dir > 0 and the far plane (pt - dir*(ZFAR-ZNEAR)) . -dir > 0.
+bool frustum_inside(const Frustum& f, float x, float y, float z) {
+    float pt[3] = {x - f.cam[0], y - f.cam[1], z - f.cam[2]};
+    bool in = true;
+    for (int k = 0; k < 4; ++k)
+        in = in && ((f.n[k][0] * pt[0] + f.n[k][1] * pt[1]) + f.n[k][2] * pt[2]) < 0.0f;
+    for (int i = 0; i < 3; ++i) pt[i] = pt[i] - f.dir[i] * kZNear;
+    in = in && ((pt[0] * f.dir[0] + pt[1] * f.dir[1]) + pt[2] * f.dir[2]) > 0.0f;
+    const float span = kZFar - kZNear;
+    for (int i = 0; i < 3; ++i) pt[i] = pt[i] - f.dir[i] * span;
+    in = in && ((pt[0] * -f.dir[0] + pt[1] * -f.dir[1]) + pt[2] * -f.dir[2]) > 0.0f;
+    return in;
+}
+
 // distance_function (src/ray_marching.cu:260-292): kNN-10 of the query (ascending), then the
 // strict minimum of sqrtf(d2) - radius; initial best is (+INF, 0).
 MarchHit sdf(const RefTree& t, const float* radii, float px, float py, float pz,
@@ -444,6 +496,13 @@ void orc_render(const void* tree, const float* radii, const float* colors, const
                 int32_t* hit, orc_counters* cnt, int nthreads) {
     orc_render_steps(tree, radii, colors, pose, pane, W, H, pix, npix, rgba, hit, nullptr, cnt,
                      nthreads);
+}
+
+void orc_frustum_inside(const float* pose, float pane, int W, int H, const float* xyzw,
+                        int64_t n, uint8_t* inside) {
+    const Frustum f = make_frustum(pose, pane, W, H);
+    for (int64_t i = 0; i < n; ++i)
+        inside[i] = frustum_inside(f, xyzw[4 * i], xyzw[4 * i + 1], xyzw[4 * i + 2]) ? 1 : 0;
 }
 
 void orc_render_steps(const void* tree, const float* radii, const float* colors,

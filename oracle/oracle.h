@@ -95,6 +95,12 @@ void orc_render_steps(const void* tree, const float* radii, const float* colors,
                       int64_t npix, float* rgba, int32_t* hit, int32_t* steps,
                       orc_counters* cnt, int nthreads);
 
+/* Frustum predicate as src/ray_marching.cu:112-124 (corner directions) and :231-258
+ * (is_inside, commented out in the reference) define it: inside[i] = 1 iff point i (xyzw,
+ * n x 4) is inside the view frustum. A culled render is orc_render over the inside points. */
+void orc_frustum_inside(const float* pose, float pane, int W, int H, const float* xyzw,
+                        int64_t n, uint8_t* inside);
+
 /* Differentiable form, fp64 reference. With the hit map fixed, per hit pixel:
  *   depth = analytic ray/sphere entry distance of the hit sphere (centre c, radius r),
  *   colour = rgb of the hit point.

@@ -61,6 +61,7 @@ SIGNATURES = {
     "rm_cloud_destroy": (_i32, [_vp]),
     "rm_cloud_set_radii": (_i32, [_vp, _vp, _vp]),
     "rm_cloud_set_colors": (_i32, [_vp, _vp, _vp]),
+    "rm_cloud_set_culling": (_i32, [_vp, _vp, _i32]),
     "rm_cloud_refit": (_i32, [_vp, _vp, _vp]),
     "rm_render": (_i32, [_vp, _vp, C.POINTER(_f32), _f32, _i32, _i32, _vp, _vp]),
     "rm_batch_create": (_i32, [_vp, _vp, C.POINTER(rm_view), _i32, C.POINTER(rm_tune), _vp,

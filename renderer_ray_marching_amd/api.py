@@ -222,6 +222,12 @@ class PointcloudRayMarcher:
     def refit(self, xyz: torch.Tensor):
         check(lib().rm_cloud_refit(self.ctx.h, self.h, _ptr(xyz.contiguous())), self.ctx.h)
 
+    def set_culling(self, enable: bool):
+        """March each view over the points inside its frustum (is_inside of
+        src/ray_marching.cu:231-258, commented out in the reference, so off by default).
+        Batches keep the setting they were created with."""
+        check(lib().rm_cloud_set_culling(self.ctx.h, self.h, 1 if enable else 0), self.ctx.h)
+
     def close(self):
         if self.h:
             lib().rm_cloud_destroy(self.h)

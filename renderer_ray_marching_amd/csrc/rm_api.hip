@@ -49,6 +49,8 @@ struct rm_cloud {
     int64_t n = 0;
     float4* xyzr = nullptr;
     float4* colors = nullptr;
+    int32_t cull = 0;
+    uint32_t* mask = nullptr;      // culling mask of single-view renders (allocated with culling)
 };
 
 struct rm_batch {
@@ -65,6 +67,8 @@ struct rm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int blocks = 0;
+    int32_t cull = 0;              // the cloud's culling setting at creation
+    uint32_t* d_mask = nullptr;    // culling masks, one per view
 };
 
 namespace {
@@ -96,16 +100,48 @@ DevTree dev_tree(const rm_tree* t) {
     return d;
 }
 
+rm::TreeLevels tree_levels(const rm_tree* t) {
+    return rm::TreeLevels{t->level_order, t->level_start.data(), (int)t->level_start.size() - 1};
+}
+
 DevCloud dev_cloud(const rm_cloud* c) {
     DevCloud d;
     d.tree = dev_tree(c->tree);
     d.xyzr = c->xyzr;
     d.colors = c->colors;
+    d.cull = c->cull;
     return d;
 }
 
 // Pane distance of src/ray_marching.cu:74, computed once on the host (float, libm tanf).
 float pane_distance(int W, float fov) { return (float)W / (2.0f * std::tan(0.5f * fov)); }
+
+// Frustum of one view, as src/ray_marching.cu:112-124 builds its edge points (and the commented
+// predicate at :243-257 would use them): corner directions E_k = M * (x W/2, y H/2, -pane, 0)
+// for (x, y) = (-1,-1), (1,-1), (1,1), (-1,1), with glm's (m0 x + m1 y) + (m2 z + m3 0); side
+// normals cross(E_k, E_{k+1 mod 4}); forward = -M[2]. Float, evaluated as written.
+void view_frustum(const float* M, float pane, int W, int H, float* fn, float* fdir) {
+    static const float cx[4] = {-1.0f, 1.0f, 1.0f, -1.0f}, cy[4] = {-1.0f, -1.0f, 1.0f, 1.0f};
+    float E[4][3];
+    for (int k = 0; k < 4; ++k) {
+        const float vx = cx[k] * (0.5f * (float)W), vy = cy[k] * (0.5f * (float)H), vz = -pane;
+        for (int i = 0; i < 3; ++i) {
+            const float add0 = M[0 + i] * vx + M[4 + i] * vy;
+            const float add1 = M[8 + i] * vz + M[12 + i] * 0.0f;
+            E[k][i] = add0 + add1;
+        }
+    }
+    for (int k = 0; k < 4; ++k) {
+        const float* a = E[k];
+        const float* b = E[(k + 1) % 4];
+        fn[3 * k + 0] = a[1] * b[2] - a[2] * b[1];
+        fn[3 * k + 1] = a[2] * b[0] - a[0] * b[2];
+        fn[3 * k + 2] = a[0] * b[1] - a[1] * b[0];
+    }
+    fdir[0] = -M[8];
+    fdir[1] = -M[9];
+    fdir[2] = -M[10];
+}
 
 DevView dev_view(const float* pose, float fov, int W, int H, int64_t pix_offset, int tile_w,
                  int64_t tile_offset) {
@@ -117,6 +153,7 @@ DevView dev_view(const float* pose, float fov, int W, int H, int64_t pix_offset,
     v.tiles_x = tile_w > 0 ? (W + tile_w - 1) / tile_w : 0;
     v.pix_offset = pix_offset;
     v.tile_offset = tile_offset;
+    view_frustum(v.M, v.pane, W, H, v.fn, v.fdir);
     return v;
 }
 
@@ -492,6 +529,7 @@ int rm_cloud_destroy(rm_cloud* c) {
     if (c->tree) (void)hipSetDevice(c->tree->device);
     if (c->xyzr) (void)hipFree(c->xyzr);
     if (c->colors) (void)hipFree(c->colors);
+    if (c->mask) (void)hipFree(c->mask);
     delete c;
     return RM_OK;
 }
@@ -507,6 +545,25 @@ int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* c, const float* colors) {
     if (!ctx || !c || !colors) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_colors: null input");
     if (int r = set_device(ctx)) return r;
     RM_HIP(ctx, rm::launch_set_colors(ctx->stream, colors, c->n, c->colors));
+    return RM_OK;
+}
+
+int rm_cloud_set_culling(rm_ctx* ctx, rm_cloud* c, int enable) {
+    if (!ctx || !c) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_culling: null input");
+    if (enable != 0 && enable != 1)
+        return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_culling: enable must be 0 or 1");
+    if (enable && !c->mask) {
+        if (int r = set_device(ctx)) return r;
+        const size_t bytes =
+            sizeof(uint32_t) * (size_t)rm::cull_view_words(c->n, c->tree->n_nodes);
+        if (hipMalloc(&c->mask, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            c->mask = nullptr;
+            return fail(ctx, RM_ERR_NOMEM, "rm_cloud_set_culling: device allocation failed");
+        }
+        RM_HIP(ctx, hipMemsetAsync(c->mask, 0, bytes, ctx->stream));
+    }
+    c->cull = enable;
     return RM_OK;
 }
 
@@ -534,7 +591,7 @@ int rm_render(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov, int 
     timing_begin(ctx);
     RM_HIP(ctx, rm::launch_render_plain(ctx->stream, dev_cloud(c), v,
                                         rm::MarchOut{reinterpret_cast<float4*>(rgba), hit},
-                                        ctx->d_err));
+                                        ctx->d_err, c->mask, tree_levels(c->tree)));
     timing_end(ctx);
     return RM_OK;
 }
@@ -598,6 +655,9 @@ int batch_layout(rm_batch* b, int view_stride = 1) {
 rm::BatchParams batch_params(rm_batch* b) {
     rm::BatchParams p;
     p.cloud = dev_cloud(b->cloud);
+    p.cloud.cull = b->cull;
+    p.cull_mask = b->d_mask;
+    p.cull_words = rm::cull_view_words(b->cloud->n, b->cloud->tree->n_nodes);
     p.views = b->d_views;
     p.nviews = (int32_t)b->laid_views;
     p.tile_w = b->tune.tile_w;
@@ -613,6 +673,11 @@ rm::BatchParams batch_params(rm_batch* b) {
 
 int batch_enqueue(rm_batch* b, hipStream_t s) {
     rm_ctx* ctx = b->ctx;
+    if (b->cull)
+        RM_HIP(ctx, rm::launch_cull_masks(
+                        s, dev_tree(b->cloud->tree), b->d_views, b->laid_views, b->d_mask,
+                        rm::cull_view_words(b->cloud->n, b->cloud->tree->n_nodes),
+                        tree_levels(b->cloud->tree)));
     RM_HIP(ctx, rm::launch_reset_queue(s, b->d_queue));
     RM_HIP(ctx, rm::launch_batch(s, batch_params(b), b->tune.mode, b->tune.warm,
                                  b->tune.waves_per_block, b->blocks));
@@ -671,8 +736,13 @@ int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
     b->tune = t;
     b->rgba = rgba;
     b->hit = hit;
+    b->cull = cloud->cull;
     if (hipMalloc(&b->d_views, sizeof(DevView) * nviews) != hipSuccess ||
-        hipMalloc(&b->d_queue, sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&b->d_queue, sizeof(unsigned long long)) != hipSuccess ||
+        (b->cull && hipMalloc(&b->d_mask, sizeof(uint32_t) * (size_t)nviews *
+                                              (size_t)rm::cull_view_words(
+                                                  cloud->n, cloud->tree->n_nodes)) !=
+                        hipSuccess)) {
         (void)hipGetLastError();
         rm_batch_destroy(b);
         return fail(ctx, RM_ERR_NOMEM, "rm_batch_create: device allocation failed");
@@ -826,6 +896,7 @@ int rm_batch_destroy(rm_batch* b) {
     batch_drop_graph(b);
     if (b->d_views) (void)hipFree(b->d_views);
     if (b->d_queue) (void)hipFree(b->d_queue);
+    if (b->d_mask) (void)hipFree(b->d_mask);
     delete b;
     return RM_OK;
 }
@@ -857,7 +928,8 @@ int rm_render_diff(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov,
     DevView v = dev_view(pose, fov, W, H, 0, 0, 0);
     DevCloud dc = dev_cloud(c);
     timing_begin(ctx);
-    RM_HIP(ctx, rm::launch_render_plain(ctx->stream, dc, v, rm::MarchOut{marched, hit}, ctx->d_err));
+    RM_HIP(ctx, rm::launch_render_plain(ctx->stream, dc, v, rm::MarchOut{marched, hit}, ctx->d_err,
+                                        c->mask, tree_levels(c->tree)));
     RM_HIP(ctx, rm::launch_diff_forward(ctx->stream, dc, v, marched, hit, rgb, depth));
     timing_end(ctx);
     return RM_OK;

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "rm_tree_host.h"
@@ -39,6 +40,7 @@ struct DevCloud {
     DevTree tree;
     const float4* xyzr;     // original order: x, y, z, radius
     const float4* colors;   // original order: r, g, b, 1
+    int32_t cull;           // 1: march over the points inside the view frustum only
 };
 
 // One view as the kernels see it.
@@ -49,7 +51,13 @@ struct DevView {
     int32_t tiles_x;
     int64_t pix_offset;     // first pixel in the concatenated outputs
     int64_t tile_offset;    // first tile in the batch's tile numbering
+    // World-space frustum: normals of the four side planes through the camera origin (x, y, z
+    // each; a point p is inside plane i iff fn_i . (p - origin) < 0), then the forward direction.
+    float fn[12];
+    float fdir[3];
 };
+static_assert(offsetof(DevView, fdir) == offsetof(DevView, fn) + 12 * sizeof(float),
+              "the kernels read fn and fdir as one run of 15 floats");
 
 struct MarchOut {
     float4* rgba;
@@ -66,6 +74,24 @@ struct BatchParams {
     unsigned long long* queue;  // work counter (tiles, or pixels with refill)
     int* err;               // device error flag
     MarchOut out;
+    const uint32_t* cull_mask;  // cloud.cull: the culling data of each view, cull_words apart
+    int64_t cull_words;
+};
+
+// Culling data of one view: a point mask (bit i: point i of the leaf order is inside the view
+// frustum; 32 points a word, whole pairs of words plus a spare pair), then one byte per inner
+// node (bit s: child slot s holds a point inside the frustum), padded to whole words.
+__host__ __device__ inline int64_t cull_mask_words(int64_t n) { return 2 * ((n + 63) / 64) + 2; }
+inline int64_t cull_view_words(int64_t n, int64_t nodes) {
+    return cull_mask_words(n) + (nodes + 3) / 4 + 1;
+}
+
+// Levels of the inner nodes, for bottom-up passes: ids of level l are
+// order[start[l] .. start[l + 1]) (order on the device, start on the host).
+struct TreeLevels {
+    const int32_t* order;
+    const int32_t* start;
+    int nlevels;
 };
 
 // Kernel launchers (rm_kernels.hip). Each returns the hipError_t of the launch.
@@ -80,10 +106,12 @@ hipError_t launch_make_cloud(hipStream_t s, const float* xyzw, const float* radi
 hipError_t launch_set_radii(hipStream_t s, const float* radii, int64_t n, float4* xyzr);
 hipError_t launch_set_colors(hipStream_t s, const float* colors, int64_t n, float4* col);
 hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& v, MarchOut o,
-                               int* err);
+                               int* err, uint32_t* cull, const TreeLevels& lv);
 hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
                         int blocks);
 hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q);
+hipError_t launch_cull_masks(hipStream_t s, const DevTree& t, const DevView* views, int nviews,
+                             uint32_t* cull, int64_t stride, const TreeLevels& lv);
 hipError_t launch_refit(hipStream_t s, RmNode* nodes, float4* pts, float4* xyzr,
                         const float* xyz, int64_t n, const int32_t* level_order,
                         const int32_t* level_start, int nlevels);

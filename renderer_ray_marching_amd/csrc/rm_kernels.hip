@@ -19,6 +19,7 @@ namespace {
 
 typedef const __attribute__((address_space(4))) float CFloat;
 typedef const __attribute__((address_space(4))) int32_t CInt;
+typedef const __attribute__((address_space(4))) uint32_t CUint;
 
 constexpr uint64_t kPadKey = (uint64_t(0x7f800000u) << 32) | 0xffffffffu;
 constexpr float kInf = __builtin_huge_valf();
@@ -48,6 +49,14 @@ __device__ __forceinline__ float mindist(float lx, float ly, float lz, float hx,
     float ey = fmaxf(fmaxf(ly - qy, qy - hy), 0.0f);
     float ez = fmaxf(fmaxf(lz - qz, qz - hz), 0.0f);
     return __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+}
+
+// Frustum culling masks: bit i of a view's mask is set iff point i of the tree (leaf order) is
+// inside the view frustum. 32-bit words; every mask carries spare words so that two consecutive
+// words can be read at any point's word.
+template <class P>
+__device__ __forceinline__ bool mask_bit(P m, int64_t i) {
+    return (m[i >> 5] >> (i & 31)) & 1u;
 }
 
 // Ascending top-K of 64-bit keys held in registers (fully unrolled).
@@ -144,13 +153,21 @@ __device__ __forceinline__ int child_entry(int ni, int s, int ref, int cnt) {
 // ------------------------------------------------------------------------------------------
 // amask: ballot of the active lanes (a wave-uniform 64-bit mask). A compare's ballot masked with
 // it needs no per-lane boolean (ballot of (active && cond) would materialise one in a VGPR).
-template <int K>
+template <int K, bool CULL = false>
 __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float qx, float qy,
                                                      float qz, uint64_t amask, const TopK<K>& tk,
-                                                     int* wstk, int& sp, int* err) {
+                                                     int* wstk, int& sp, int* err,
+                                                     CUint* nbits = nullptr) {
     CFloat* f = nf + (int64_t)ni * 16;
     CInt* fi = (CInt*)f;
-    const int r0 = fi[12], r1 = fi[13], c0 = fi[14], c1 = fi[15];
+    const int r0 = fi[12], r1 = fi[13];
+    int c0 = fi[14], c1 = fi[15];
+    if (CULL) {
+        // a child slot with no point inside the view frustum is treated as empty
+        const uint32_t nb = nbits[ni >> 2] >> ((ni & 3) * 8);
+        if (!(nb & 1u)) c0 = -1;
+        if (!(nb & 2u)) c1 = -1;
+    }
     const float w = tk.worst_d2();
     const float m0 = mindist(f[0], f[1], f[2], f[3], f[4], f[5], qx, qy, qz);
     const float m1 = mindist(f[6], f[7], f[8], f[9], f[10], f[11], qx, qy, qz);
@@ -174,12 +191,19 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
     }
 }
 
-template <int K, bool NODUP>
+template <int K, bool NODUP, bool CULL = false>
 __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, float qx, float qy,
-                                                 float qz, TopK<K>& tk) {
+                                                 float qz, TopK<K>& tk, CUint* mask = nullptr) {
     CFloat* p = pf + (int64_t)b * 4;
     int j = 0;
     for (; j + 4 <= cnt; j += 4) {
+        // culled points (wave-uniform bits) are skipped by the scalar unit
+        uint32_t keep = 0xfu;
+        if (CULL) {
+            const int64_t i0 = (int64_t)b + j;
+            const uint64_t win = (uint64_t)mask[i0 >> 5] | ((uint64_t)mask[(i0 >> 5) + 1] << 32);
+            keep = (uint32_t)(win >> (i0 & 31)) & 0xfu;
+        }
         float x[4], y[4], z[4];
         uint32_t id[4];
 #pragma unroll
@@ -196,10 +220,11 @@ __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, flo
         // and the key is formed and compared only when some lane passes it
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (__builtin_amdgcn_ballot_w64(d[u] <= tk.worst_d2()) != 0)
+            if (((keep >> u) & 1u) && __builtin_amdgcn_ballot_w64(d[u] <= tk.worst_d2()) != 0)
                 tk.template add<NODUP>(make_key(d[u], id[u]));
     }
     for (; j < cnt; ++j) {
+        if (CULL && !mask_bit(mask, (int64_t)b + j)) continue;
         float d = dist2(p[4 * j], p[4 * j + 1], p[4 * j + 2], qx, qy, qz);
         tk.template add<NODUP>(make_key(d, __float_as_uint(p[4 * j + 3])));
     }
@@ -243,14 +268,16 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
 // the scan loop only, so it keeps its registers across the node expansions. Leaves are queued
 // in the near-first order of the depth-first walk; bounds tightened by a round prune the
 // entries still on the stack when they are expanded.
-template <int K, bool NODUP = false>
+template <int K, bool NODUP = false, bool CULL = false>
 __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz, bool active,
-                                  TopK<K>& tk, int* wstk, int* lq, int* err) {
+                                  TopK<K>& tk, int* wstk, int* lq, int* err,
+                                  CUint* mask = nullptr) {
     CFloat* nf = (CFloat*)t.nodes;
     CFloat* pf = (CFloat*)t.pts;
+    CUint* nbits = CULL ? mask + cull_mask_words(t.n) : nullptr;
     const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
     int sp = 0;
-    packet_push_children<K>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err);
+    packet_push_children<K, CULL>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err, nbits);
     while (sp > 0) {
         int nq = 0;
         while (sp > 0 && nq < kLeafQueue) {
@@ -259,7 +286,8 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
             if (e & 1)
                 lq[nq++] = e;
             else
-                packet_push_children<K>(nf, e >> 1, qx, qy, qz, amask, tk, wstk, sp, err);
+                packet_push_children<K, CULL>(nf, e >> 1, qx, qy, qz, amask, tk, wstk, sp, err,
+                                              nbits);
         }
         for (int i = 0; i < nq; ++i) {
             const int e = __builtin_amdgcn_readfirstlane(lq[i]);
@@ -269,7 +297,7 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
             CFloat* bx = f + 6 * s;
             const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
             if ((__builtin_amdgcn_ballot_w64(md <= tk.worst_d2()) & amask) == 0) continue;
-            packet_scan_leaf<K, NODUP>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk);
+            packet_scan_leaf<K, NODUP, CULL>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk, mask);
         }
     }
 }
@@ -293,17 +321,23 @@ __device__ __forceinline__ NodeRec load_node(const RmNode* nodes, int ni) {
     return n;
 }
 
-template <class Set>
+template <bool CULL = false, class Set>
 __device__ __forceinline__ void lane_push_children(const RmNode* nodes, int ni, float qx,
                                                    float qy, float qz, const Set& tk,
                                                    int* lstk, int stride, int& sp, int maxsp,
-                                                   int* err) {
+                                                   int* err, const uint8_t* nbits = nullptr) {
     NodeRec n = load_node(nodes, ni);
     const float w = tk.worst_d2();
     const float m0 = mindist(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, qx, qy, qz);
     const float m1 = mindist(n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, qx, qy, qz);
-    const bool n0 = (n.r.z >= 0) & (m0 <= w);
-    const bool n1 = (n.r.w >= 0) & (m1 <= w);
+    bool n0 = (n.r.z >= 0) & (m0 <= w);
+    bool n1 = (n.r.w >= 0) & (m1 <= w);
+    if (CULL) {
+        // a child slot with no point inside the view frustum is not entered
+        const uint32_t nb = nbits[ni];
+        n0 = n0 & ((nb & 1u) != 0);
+        n1 = n1 & ((nb & 2u) != 0);
+    }
     if (sp + 2 > maxsp) {
         if (n0 || n1) *err = 1;
         return;
@@ -320,15 +354,17 @@ __device__ __forceinline__ void lane_push_children(const RmNode* nodes, int ni, 
     }
 }
 
-template <bool NODUP = false, class Set>
+template <bool NODUP = false, bool CULL = false, class Set>
 __device__ void knn_lane(const DevTree& t, float qx, float qy, float qz, Set& tk, int* lstk,
-                         int stride, int maxsp, int* err) {
+                         int stride, int maxsp, int* err, const uint32_t* mask = nullptr) {
     int sp = 0;
-    lane_push_children(t.nodes, 0, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
+    const uint8_t* nbits = CULL ? (const uint8_t*)(mask + cull_mask_words(t.n)) : nullptr;
+    lane_push_children<CULL>(t.nodes, 0, qx, qy, qz, tk, lstk, stride, sp, maxsp, err, nbits);
     while (sp > 0) {
         const int e = lstk[(--sp) * stride];
         if (!(e & 1)) {
-            lane_push_children(t.nodes, e >> 1, qx, qy, qz, tk, lstk, stride, sp, maxsp, err);
+            lane_push_children<CULL>(t.nodes, e >> 1, qx, qy, qz, tk, lstk, stride, sp, maxsp,
+                                     err, nbits);
             continue;
         }
         const int ni = e >> 2, s = (e >> 1) & 1;
@@ -346,13 +382,15 @@ __device__ void knn_lane(const DevTree& t, float qx, float qy, float qz, Set& tk
             for (int u = 0; u < 4; ++u) v[u] = p[j + u];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                tk.template add<NODUP>(make_key(dist2(v[u].x, v[u].y, v[u].z, qx, qy, qz),
-                                                __float_as_uint(v[u].w)));
+                if (!CULL || mask_bit(mask, (int64_t)ref + j + u))
+                    tk.template add<NODUP>(make_key(dist2(v[u].x, v[u].y, v[u].z, qx, qy, qz),
+                                                    __float_as_uint(v[u].w)));
         }
         for (; j < cnt; ++j) {
             float4 v = p[j];
-            tk.template add<NODUP>(
-                make_key(dist2(v.x, v.y, v.z, qx, qy, qz), __float_as_uint(v.w)));
+            if (!CULL || mask_bit(mask, (int64_t)ref + j))
+                tk.template add<NODUP>(
+                    make_key(dist2(v.x, v.y, v.z, qx, qy, qz), __float_as_uint(v.w)));
         }
     }
 }
@@ -541,13 +579,19 @@ __device__ __forceinline__ void sdf10(const DevCloud& c, const TopK<kMarchK>& tk
 // Warm start: the previous step's 10 neighbours, re-measured from the new query point and
 // sorted, seed the list. They are 10 valid candidates, so the list's 10th key bounds the new 10th
 // key from above and pruning is tight from the first node; the traversal then only inserts
-// points not already listed (insert_nodup), and its result is the exact top 10. Needs n >= 10.
+// points not already listed (insert_nodup), and its result is the exact top 10. Needs n >= 10;
+// with culling fewer than 10 points may be visible, and padding entries stay padding.
+template <bool CULL = false>
 __device__ __forceinline__ void warm_seed(const DevCloud& c, TopK<kMarchK>& tk, float qx,
                                           float qy, float qz) {
     const float* xyzr = (const float*)c.xyzr;
 #pragma unroll
     for (int j = 0; j < kMarchK; ++j) {
         const uint32_t id = key_idx(tk.key[j]);
+        if (CULL && id >= (uint32_t)c.tree.n) {
+            tk.key[j] = kPadKey;
+            continue;
+        }
         const float* q = xyzr + 4 * (size_t)id;   // x, y, z only: the radius is not needed
         tk.key[j] = make_key(dist2(q[0], q[1], q[2], qx, qy, qz), id);
     }
@@ -555,9 +599,9 @@ __device__ __forceinline__ void warm_seed(const DevCloud& c, TopK<kMarchK>& tk, 
     sort10(tk.key);
 }
 
-template <bool WARM>
-__device__ void march_lane(const DevCloud& c, const Ray& r, int* lstk, int stride, int maxsp,
-                           int* err, int& hit, float& depth) {
+template <bool WARM, bool CULL>
+__device__ void march_lane(const DevCloud& c, const Ray& r, const uint32_t* mask, int* lstk,
+                           int stride, int maxsp, int* err, int& hit, float& depth) {
     TopK<kMarchK> tk;
     tk.init(kPadKey);
     float t = r.tnear;
@@ -568,11 +612,11 @@ __device__ void march_lane(const DevCloud& c, const Ray& r, int* lstk, int strid
         float py = r.oy + r.dy * t;
         float pz = r.oz + r.dz * t;
         if (WARM && i > 0) {
-            warm_seed(c, tk, px, py, pz);
-            knn_lane<true>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
+            warm_seed<CULL>(c, tk, px, py, pz);
+            knn_lane<true, CULL>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err, mask);
         } else {
             tk.init(kPadKey);
-            knn_lane(c.tree, px, py, pz, tk, lstk, stride, maxsp, err);
+            knn_lane<false, CULL>(c.tree, px, py, pz, tk, lstk, stride, maxsp, err, mask);
         }
         float s;
         uint32_t id;
@@ -599,9 +643,12 @@ __device__ __forceinline__ void write_pixel(const DevCloud& c, MarchOut o, int64
 }
 
 // Plain per-view path: the launch shape of src/ray_marching.cu:147-149 (16x16 threads, one
-// pixel each), per-lane traversal restarted from the root at every step.
+// pixel each), per-lane traversal restarted from the root at every step. With CULL the kNN
+// runs over the points of the view's culling mask only.
+template <bool CULL>
 __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v, MarchOut o,
-                                                           int maxsp, int* err) {
+                                                           int maxsp, int* err,
+                                                           const uint32_t* mask) {
     extern __shared__ int lstk[];
     int x = blockIdx.x * 16 + threadIdx.x;
     int y = blockIdx.y * 16 + threadIdx.y;
@@ -610,7 +657,7 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
     Ray r = make_ray([&](int i) { return v.M[i]; }, v.pane, v.W, v.H, x, y);
     int hit;
     float depth;
-    march_lane<false>(c, r, &lstk[tid], 256, maxsp, err, hit, depth);
+    march_lane<false, CULL>(c, r, mask, &lstk[tid], 256, maxsp, err, hit, depth);
     write_pixel(c, o, v.pix_offset + (int64_t)y * v.W + x, hit, depth);
 }
 
@@ -627,7 +674,7 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
 // compiled for 8 wavefronts per SIMD (64 VGPRs); the values that do not fit are per-ray state
 // spilled to scratch outside the traversal loops (measured 17 % faster than 5 wavefronts per SIMD
 // at 91 VGPRs without spills; the vector ALUs then issue 90 % of the time).
-template <int MODE, bool WARM, int WPB>
+template <int MODE, bool WARM, int WPB, bool CULL>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) void batch_kernel(
     BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
@@ -670,6 +717,7 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
         const int y0 = (int)(lt / tiles_x) * p.tile_h;
         // camera origin: the same for every lane of the tile (scalar registers)
         const float ox = V[12], oy = V[13], oz = V[14];
+        const uint32_t* vmask = CULL ? p.cull_mask + (int64_t)lo * p.cull_words : nullptr;
         int next = 0;                      // wave-uniform: next unclaimed pixel of the tile
         bool has = false;
         // per-lane ray state: direction, far limit, distance marched, steps, pixel in the view
@@ -709,21 +757,21 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
             if (!has)
                 tk.init(0ull);
             else if (warm_lane)
-                warm_seed(p.cloud, tk, px, py, pz);
+                warm_seed<CULL>(p.cloud, tk, px, py, pz);
             else
                 tk.init(kPadKey);
             if (MODE == 0) {
                 // a fresh lane's pads (index 0xffffffff) never match a real candidate, so the
                 // duplicate check is harmless for it and one traversal serves both kinds
-                knn_packet_rounds<kMarchK, WARM>(p.cloud.tree, px, py, pz, has, tk, wstk[w], lq[w],
-                                                 p.err);
+                knn_packet_rounds<kMarchK, WARM, CULL>(p.cloud.tree, px, py, pz, has, tk, wstk[w],
+                                                       lq[w], p.err, (CUint*)vmask);
             } else if (has) {
                 if (warm_lane)
-                    knn_lane<true>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
-                                            WPB * 64, maxsp, p.err);
+                    knn_lane<true, CULL>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
+                                         WPB * 64, maxsp, p.err, vmask);
                 else
-                    knn_lane(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
-                                      WPB * 64, maxsp, p.err);
+                    knn_lane<false, CULL>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
+                                          WPB * 64, maxsp, p.err, vmask);
             }
             if (has) {
                 float s;
@@ -740,6 +788,94 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
             }
         }
     }
+}
+
+// Frustum predicate is_inside of src/ray_marching.cu:231-258 (commented out there, so the
+// reference never culls): the point is inside the four side planes through the camera origin
+// (o), beyond the near plane and before the far plane. F(i) yields the view's frustum floats: side normals 0..11, forward
+// 12..14. Float, no contraction, sums left to right as written.
+template <class FR>
+__device__ __forceinline__ bool frustum_inside(FR F, float ox, float oy, float oz, float x,
+                                               float y, float z) {
+    const float px = x - ox, py = y - oy, pz = z - oz;
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        in &= ((F(3 * i) * px + F(3 * i + 1) * py) + F(3 * i + 2) * pz) < 0.0f;
+    const float fx = F(12), fy = F(13), fz = F(14);
+    const float ax = px - fx * kZNear, ay = py - fy * kZNear, az = pz - fz * kZNear;
+    in &= ((ax * fx + ay * fy) + az * fz) > 0.0f;
+    const float bx = ax - fx * (kZFar - kZNear), by = ay - fy * (kZFar - kZNear),
+                bz = az - fz * (kZFar - kZNear);
+    in &= ((bx * -fx + by * -fy) + bz * -fz) > 0.0f;
+    return in;
+}
+
+// Point masks of nviews views (MANY: the device view table; otherwise the one view `one`):
+// one wavefront per 64 consecutive leaf-ordered points, the predicate of every lane's point
+// packed by ballot into two mask words per view. View v's data starts at mask + v * stride.
+template <bool MANY>
+__global__ __launch_bounds__(256) void cull_mask_kernel(DevTree t, DevView one,
+                                                        const DevView* views, int nviews,
+                                                        uint32_t* __restrict__ mask,
+                                                        int64_t stride) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t base = i - lane_id();
+    if (base >= t.n) return;
+    const bool valid = i < t.n;
+    const float4 q = valid ? t.pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int v = 0; v < nviews; ++v) {
+        bool in = false;
+        if (MANY) {
+            CFloat* V = (CFloat*)(views + v);
+            CFloat* F = V + offsetof(DevView, fn) / 4;
+            in = frustum_inside([&](int k) { return F[k]; }, V[12], V[13], V[14], q.x, q.y, q.z);
+        } else {
+            in = frustum_inside([&](int k) { return k < 12 ? one.fn[k] : one.fdir[k - 12]; },
+                                one.M[12], one.M[13], one.M[14], q.x, q.y, q.z);
+        }
+        const uint64_t b = __builtin_amdgcn_ballot_w64(valid && in);
+        if (lane_id() < 2) {
+            const uint32_t w = lane_id() == 0 ? (uint32_t)b : (uint32_t)(b >> 32);
+            mask[(int64_t)v * stride + (base >> 5) + lane_id()] = w;
+        }
+    }
+}
+
+// Node bits of one level of inner nodes for every view (blockIdx.y = view), bottom-up: a leaf
+// child's bit is the OR of its points' mask bits, an inner child's the OR of its own two bits
+// (written by the launch for the level below).
+__global__ __launch_bounds__(256) void cull_nodes_kernel(const RmNode* __restrict__ nodes,
+                                                         const int32_t* __restrict__ ids,
+                                                         int count, int64_t n,
+                                                         uint32_t* __restrict__ cull,
+                                                         int64_t stride) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t* pm = cull + (int64_t)blockIdx.y * stride;
+    uint8_t* nb = (uint8_t*)(cull + (int64_t)blockIdx.y * stride + cull_mask_words(n));
+    const int id = ids[i];
+    const RmNode& nd = nodes[id];
+    uint32_t bits = 0;
+    for (int s = 0; s < 2; ++s) {
+        const int c = nd.cnt[s];
+        if (c < 0) continue;
+        bool any = false;
+        if (c > 0) {
+            const int64_t b = nd.ref[s], e = b + c;
+            for (int64_t w = b >> 5; w <= (e - 1) >> 5; ++w) {
+                const int64_t lo = (b > w * 32 ? b : w * 32) - w * 32;
+                const int64_t hi = (e < w * 32 + 32 ? e : w * 32 + 32) - w * 32;
+                const uint32_t sel =
+                    hi - lo == 32 ? ~0u : (((1u << (uint32_t)(hi - lo)) - 1u) << (uint32_t)lo);
+                any |= (pm[w] & sel) != 0u;
+            }
+        } else {
+            any = nb[nd.ref[s]] != 0;
+        }
+        bits |= any ? (1u << s) : 0u;
+    }
+    nb[id] = (uint8_t)bits;
 }
 
 __global__ void reset_queue_kernel(unsigned long long* q) { *q = 0ull; }
@@ -1154,21 +1290,60 @@ hipError_t launch_set_colors(hipStream_t s, const float* colors, int64_t n, floa
     return hipGetLastError();
 }
 
+static hipError_t launch_cull_nodes(hipStream_t s, const DevTree& t, uint32_t* cull,
+                                    int64_t stride, int nviews, const TreeLevels& lv) {
+    for (int l = lv.nlevels - 1; l >= 0; --l) {
+        const int b = lv.start[l], cnt = lv.start[l + 1] - b;
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(cull_nodes_kernel, dim3(grid_for(cnt, 256), nviews), dim3(256), 0, s,
+                           t.nodes, lv.order + b, cnt, t.n, cull, stride);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& v, MarchOut o,
-                               int* err) {
+                               int* err, uint32_t* cull, const TreeLevels& lv) {
     int maxsp = lane_stack_depth(c.tree);
     size_t shmem = (size_t)maxsp * 256 * sizeof(int);
     dim3 grid((v.W + 15) / 16, (v.H + 15) / 16);
-    hipLaunchKernelGGL(render_plain_kernel, grid, dim3(16, 16), shmem, s, c, v, o, maxsp, err);
+    if (c.cull) {
+        if (!cull) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(cull_mask_kernel<false>, dim3(grid_for(c.tree.n, 256)), dim3(256), 0,
+                           s, c.tree, v, (const DevView*)nullptr, 1, cull, (int64_t)0);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = launch_cull_nodes(s, c.tree, cull, 0, 1, lv);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(render_plain_kernel<true>, grid, dim3(16, 16), shmem, s, c, v, o,
+                           maxsp, err, (const uint32_t*)cull);
+    } else {
+        hipLaunchKernelGGL(render_plain_kernel<false>, grid, dim3(16, 16), shmem, s, c, v, o,
+                           maxsp, err, (const uint32_t*)nullptr);
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_cull_masks(hipStream_t s, const DevTree& t, const DevView* views, int nviews,
+                             uint32_t* cull, int64_t stride, const TreeLevels& lv) {
+    if (t.n == 0 || nviews <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cull_mask_kernel<true>, dim3(grid_for(t.n, 256)), dim3(256), 0, s, t,
+                       DevView{}, views, nviews, cull, stride);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_cull_nodes(s, t, cull, stride, nviews, lv);
 }
 
 template <int MODE, bool WARM, int WPB>
 static hipError_t launch_batch_t(hipStream_t s, const BatchParams& p, int blocks) {
     int maxsp = lane_stack_depth(p.cloud.tree);
     size_t shmem = MODE == 1 ? (size_t)maxsp * WPB * 64 * sizeof(int) : 0;
-    hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB>), dim3(blocks), dim3(WPB * 64), shmem, s,
-                       p, maxsp);
+    if (p.cloud.cull)
+        hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB, true>), dim3(blocks), dim3(WPB * 64),
+                           shmem, s, p, maxsp);
+    else
+        hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB, false>), dim3(blocks), dim3(WPB * 64),
+                           shmem, s, p, maxsp);
     return hipGetLastError();
 }
 

@@ -397,6 +397,102 @@ def test_batch_autotune_keeps_results(rm_ctx, room_small):
 
 
 # ------------------------------------------------------------------------------------------
+# frustum culling (opt-in): the march over the points inside each view's frustum
+# ------------------------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def room_culled(rm_ctx):
+    c = scenes.room_scene(400_000, seed=2)
+    sizes = [(160, 120, 125.0), (213, 138, 160.0), (96, 54, 80.0), (7, 5, 5.0)]
+    views = scenes.room_views(8, seed=102, sizes=sizes)
+    t, m, rc = _marcher(rm_ctx, c, 0.05)
+    m.set_culling(True)
+    plain = []
+    for v in views:
+        rgba, hit = m.render_to_texture(v.pose_colmajor(), v.fov, v.width, v.height)
+        plain.append((rgba.reshape(-1, 4).clone(), hit.reshape(-1).clone()))
+    rm_ctx.sync()
+    return c, views, t, m, rc, plain
+
+
+def test_culled_plain_vs_oracle(rm_ctx, room_culled):
+    c, views, t, m, rc, plain = room_culled
+    pts, col, r = c.xyzw(), c.colors4(), rc.cpu().numpy()
+    changed = 0
+    for v, (rgba, hit) in zip(views, plain):
+        orgba, ohit, _ = ob.render_culled(pts, r, col, v.pose_colmajor(), v.fov, v.width,
+                                          v.height)
+        assert (hit.cpu().numpy() == ohit).all()
+        assert (rgba.cpu().numpy() == orgba).all()
+        _, uhit, _ = ob.RefTree(pts, 64).render(r, col, v.pose_colmajor(), v.fov, v.width,
+                                                 v.height)
+        changed += int((uhit != ohit).sum())
+    assert changed > 0          # culling changes pixels on these views
+
+
+@pytest.mark.parametrize("ti", [0, 1, 2, 3, 4])
+def test_culled_batch_equals_plain(rm_ctx, room_culled, ti):
+    c, views, t, m, rc, plain = room_culled
+    b = api.Batch(m, views, tune=TUNES[ti])
+    for rep in range(2):
+        b.rgba.fill_(7.0)
+        b.hit.fill_(-7)
+        b.launch()
+        rm_ctx.sync()
+        off = 0
+        for v, (prgba, phit) in zip(views, plain):
+            n = v.width * v.height
+            assert torch.equal(b.hit[off:off + n], phit)
+            assert torch.equal(b.rgba[off:off + n], prgba)
+            off += n
+    b.close()
+
+
+def test_culled_few_and_no_visible_points(rm_ctx):
+    """Fewer than 10 points inside the frustum (padding in the warm-started list) and none."""
+    fx, w, h = 40.0, 64, 48
+    fov = scenes.fov_radians(w, fx)
+    pane = ob.pane_distance(w, fov)
+    rng = np.random.default_rng(11)
+    M = scenes.look_at((0.0, 0.0, 0.0), (0.0, -1.0, 0.0))
+    # 6 points in view, 300 behind the camera
+    u = rng.uniform(-0.8, 0.8, 6) * w / 2
+    v = rng.uniform(-0.8, 0.8, 6) * h / 2
+    dep = rng.uniform(3.0, 6.0, 6)
+    cam = np.stack([u, v, -np.full(6, pane)], 1) * (dep / pane)[:, None]
+    front = cam @ M[:3, :3].T.astype(np.float64)
+    back = rng.normal(0, 1, (300, 3)) + np.array([0.0, 5.0, 0.0])
+    xyz = np.concatenate([front, back]).astype(np.float32)
+    pts = np.concatenate([xyz, np.ones((xyz.shape[0], 1), np.float32)], 1)
+    radii = np.full(pts.shape[0], 0.3, np.float32)
+    col = rng.uniform(0, 1, (pts.shape[0], 4)).astype(np.float32)
+    col[:, 3] = 1
+    tree = _tree(rm_ctx, pts)
+    mm = api.PointcloudRayMarcher(_dev(pts), _dev(col), _dev(radii), tree)
+    mm.set_culling(True)
+    # the second camera looks away from every point
+    views = [scenes.View(M, w, h, fx),
+             scenes.View(scenes.look_at((0.0, -20.0, 0.0), (0.0, -21.0, 0.0)), w, h, fx)]
+    for vw in views:
+        rgba, hit = mm.render_to_texture(vw.pose_colmajor(), vw.fov, w, h)
+        rm_ctx.sync()
+        orgba, ohit, ins = ob.render_culled(pts, radii, col, vw.pose_colmajor(), vw.fov, w, h)
+        assert (hit.cpu().numpy().ravel() == ohit).all()
+        assert (rgba.cpu().numpy().reshape(-1, 4) == orgba).all()
+    assert ob.frustum_inside(views[0].pose_colmajor(), fov, w, h, pts).sum() == 6
+    assert ob.frustum_inside(views[1].pose_colmajor(), fov, w, h, pts).sum() == 0
+    for tune in (TUNES[0], TUNES[1]):
+        b = api.Batch(mm, views, tune=tune)
+        b.launch()
+        rm_ctx.sync()
+        for k, vw in enumerate(views):
+            _, ohit, _ = ob.render_culled(pts, radii, col, vw.pose_colmajor(), vw.fov, w, h)
+            assert (b.hit[k * w * h:(k + 1) * w * h].cpu().numpy() == ohit).all()
+        b.close()
+    mm.close()
+
+
+# ------------------------------------------------------------------------------------------
 # full-size configurations: the batched march equals the plain path on every pixel and the
 # oracle on a deterministic pixel subsample
 # ------------------------------------------------------------------------------------------
