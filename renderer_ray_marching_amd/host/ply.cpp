@@ -90,6 +90,15 @@ void check_color_type(const Prop& p) {
 PlyCloud load_ply(const std::string& path) {
     std::ifstream f(path, std::ios::binary);
     if (!f) throw std::runtime_error("PLY parser: could not open file " + path);
+    f.seekg(0, std::ios::end);
+    const int64_t file_bytes = (int64_t)f.tellg();
+    f.seekg(0, std::ios::beg);
+    // bytes left after the current read position (counts in a header are checked against it
+    // before anything is allocated for them)
+    auto remaining = [&]() -> int64_t {
+        const std::streamoff pos = f.tellg();
+        return pos < 0 ? 0 : file_bytes - (int64_t)pos;
+    };
     std::string line;
     auto next_line = [&](const char* what) {
         if (!std::getline(f, line)) throw std::runtime_error(std::string("PLY parser: ") + what);
@@ -162,6 +171,19 @@ PlyCloud load_ply(const std::string& path) {
         else check_color_type(V.props[slot[k]]);
     }
 
+    // Each vertex takes at least one byte per property value in binary files and two (a digit
+    // and a separator) in ascii ones: a count the rest of the file cannot hold is rejected here,
+    // before its arrays are allocated.
+    {
+        int64_t min_bytes = 0;
+        for (const Prop& P : V.props)
+            min_bytes += fmt == Format::Ascii ? 2 : type_size(P.is_list ? P.count_type : P.type);
+        if (min_bytes > 0 && V.count > remaining() / min_bytes)
+            throw std::runtime_error(std::string("PLY parser: unexpected end of ") +
+                                     (fmt == Format::Ascii ? "ascii" : "binary") +
+                                     " data (element vertex count " +
+                                     std::to_string(V.count) + " exceeds the file)");
+    }
     PlyCloud out;
     out.n = V.count;
     out.xyzw.assign(4 * (size_t)V.count, 0.f);
@@ -215,6 +237,8 @@ PlyCloud load_ply(const std::string& path) {
     const bool swap = (fmt == Format::BinBE);
     std::vector<unsigned char> buf;
     auto need = [&](size_t bytes) {
+        if ((int64_t)bytes > remaining())
+            throw std::runtime_error("PLY parser: unexpected end of binary data");
         buf.resize(bytes);
         f.read((char*)buf.data(), (std::streamsize)bytes);
         if ((size_t)f.gcount() != bytes) throw std::runtime_error("PLY parser: unexpected end of binary data");
@@ -250,7 +274,9 @@ PlyCloud load_ply(const std::string& path) {
                 if (P.is_list) {
                     need((size_t)type_size(P.count_type));
                     const double c = read_scalar(buf.data(), P.count_type, swap);
-                    if (c < 0) throw std::runtime_error("PLY parser: negative list count");
+                    if (!(c >= 0)) throw std::runtime_error("PLY parser: negative list count");
+                    if (c * type_size(P.type) > (double)remaining())
+                        throw std::runtime_error("PLY parser: unexpected end of binary data");
                     need((size_t)c * (size_t)type_size(P.type));
                     continue;
                 }

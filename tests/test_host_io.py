@@ -157,6 +157,17 @@ def test_ply_errors(tmp_path):
         hio.load_ply(ply(good, b"\0" * 12))
     with pytest.raises(hio.RMHError, match="end of binary"):
         hio.load_ply(ply(good + [("uchar", "red"), ("uchar", "green"), ("uchar", "blue")], b"\0" * 10))
+    # a vertex count the file cannot hold is rejected before anything is allocated for it
+    # (run in a child process with a 2 GiB address-space limit)
+    colours = [("uchar", "red"), ("uchar", "green"), ("uchar", "blue")]
+    for fmt, body in (("binary_little_endian", b"\0" * 15), ("ascii", b"1 2 3 4 5 6\n")):
+        big = ply(good + colours, body, fmt=fmt, n=4_000_000_000)
+        code = ("import resource, sys; resource.setrlimit(resource.RLIMIT_AS, (2 << 30, 2 << 30)); "
+                "from renderer_ray_marching_amd import host_io as h\n"
+                "try:\n h.load_ply(sys.argv[1])\nexcept h.RMHError as e:\n print(e)")
+        r = subprocess.run([sys.executable, "-c", code, str(big)], capture_output=True, text=True,
+                           cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        assert r.returncode == 0 and "exceeds the file" in r.stdout, r.stdout + r.stderr
     bad = tmp_path / "n.ply"
     bad.write_bytes(b"plx\n")
     with pytest.raises(hio.RMHError):
