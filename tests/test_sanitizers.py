@@ -1,10 +1,12 @@
-"""Host drop-in I/O under AddressSanitizer + UndefinedBehaviorSanitizer, on the CPU.
+"""Host drop-in I/O and the CPU oracle under AddressSanitizer + UndefinedBehaviorSanitizer.
 
 The parsers of librm_host (PLY, the .kdtree.radii text archive, the matrices JSON) and its
 writers are compiled with g++ -fsanitize=address,undefined together with
 tests/sanitize/host_io_fuzz.cpp, which feeds every parser its valid input and thousands of
 deterministic corruptions of it (truncations, byte changes). Each must be parsed or rejected with
-an error code; any sanitizer report or crash fails the test.
+an error code; any sanitizer report or crash fails the test. The oracle is built the same way
+with tests/sanitize/oracle_driver.cpp, which calls each of its entry points on small edge-case
+clouds. CPU only; no GPU code is built with a sanitizer.
 """
 import json
 import os
@@ -94,3 +96,18 @@ def test_matrices_parser_under_sanitizers(fuzzer, tmp_path):
 
 def test_writers_and_paths_under_sanitizers(fuzzer, tmp_path):
     assert "io: ok" in _run(fuzzer, "io", "-", tmp_path)
+
+
+def test_oracle_under_sanitizers(tmp_path):
+    out = str(tmp_path / "oracle_driver")
+    oracle = os.path.join(ROOT, "oracle")
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fopenmp", "-ffp-contract=off", "-mfma",
+           "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-I", oracle, os.path.join(oracle, "oracle.cpp"),
+           os.path.join(ROOT, "tests", "sanitize", "oracle_driver.cpp"), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", OMP_NUM_THREADS="2")
+    r = subprocess.run([out], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0 and "oracle: ok" in r.stdout, (r.stdout + r.stderr)[-4000:]
+    assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
