@@ -110,7 +110,9 @@ int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* cloud, const float* colors);
  * src/ray_marching.cu:231-258 with the edge points of :112-124, which the reference computes but
  * has commented out. The kNN-10 of every march step is then exact over the inside points.
  * Applies to rm_render, rm_render_diff and to batches created after the call (a batch keeps the
- * setting it was created with). Enabling allocates one view's culling data on the cloud. */
+ * setting it was created with). Enabling allocates one view's culling data on the cloud, which
+ * single-view renders share: culled rm_render / rm_render_diff calls on one cloud must be
+ * ordered (same stream), while each batch has its own. */
 int rm_cloud_set_culling(rm_ctx* ctx, rm_cloud* cloud, int enable);
 /* Move the points of a cloud (xyz: n x 3 floats, original order) and refit the boxes of the
  * tree it was created from, keeping the topology. Search stays exact because every bound is
