@@ -51,6 +51,7 @@ struct rm_cloud {
     float4* colors = nullptr;
     int32_t cull = 0;
     uint32_t* mask = nullptr;      // culling mask of single-view renders (allocated with culling)
+    uint32_t* rmax = nullptr;      // device: bits of the largest radius
 };
 
 struct rm_batch {
@@ -110,6 +111,7 @@ DevCloud dev_cloud(const rm_cloud* c) {
     d.xyzr = c->xyzr;
     d.colors = c->colors;
     d.cull = c->cull;
+    d.rmax = c->rmax;
     return d;
 }
 
@@ -510,12 +512,14 @@ int rm_cloud_create(rm_ctx* ctx, const rm_tree* tree, const float* xyzw, const f
     c->tree = const_cast<rm_tree*>(tree);
     c->n = n;
     if (hipMalloc(&c->xyzr, sizeof(float4) * (size_t)n) != hipSuccess ||
-        hipMalloc(&c->colors, sizeof(float4) * (size_t)n) != hipSuccess) {
+        hipMalloc(&c->colors, sizeof(float4) * (size_t)n) != hipSuccess ||
+        hipMalloc(&c->rmax, sizeof(uint32_t)) != hipSuccess) {
         (void)hipGetLastError();
         rm_cloud_destroy(c);
         return fail(ctx, RM_ERR_NOMEM, "rm_cloud_create: device allocation failed");
     }
     hipError_t e = rm::launch_make_cloud(ctx->stream, xyzw, radii, colors, n, c->xyzr, c->colors);
+    if (e == hipSuccess) e = rm::launch_max_radius(ctx->stream, c->xyzr, n, c->rmax);
     if (e != hipSuccess) {
         rm_cloud_destroy(c);
         return hip_fail(ctx, e, "rm_cloud_create");
@@ -530,6 +534,7 @@ int rm_cloud_destroy(rm_cloud* c) {
     if (c->xyzr) (void)hipFree(c->xyzr);
     if (c->colors) (void)hipFree(c->colors);
     if (c->mask) (void)hipFree(c->mask);
+    if (c->rmax) (void)hipFree(c->rmax);
     delete c;
     return RM_OK;
 }
@@ -538,6 +543,7 @@ int rm_cloud_set_radii(rm_ctx* ctx, rm_cloud* c, const float* radii) {
     if (!ctx || !c || !radii) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_radii: null input");
     if (int r = set_device(ctx)) return r;
     RM_HIP(ctx, rm::launch_set_radii(ctx->stream, radii, c->n, c->xyzr));
+    RM_HIP(ctx, rm::launch_max_radius(ctx->stream, c->xyzr, c->n, c->rmax));
     return RM_OK;
 }
 

@@ -41,6 +41,7 @@ struct DevCloud {
     const float4* xyzr;     // original order: x, y, z, radius
     const float4* colors;   // original order: r, g, b, 1
     int32_t cull;           // 1: march over the points inside the view frustum only
+    const uint32_t* rmax;   // bits of the largest radius (at least 0), kept on the device
 };
 
 // One view as the kernels see it.
@@ -104,6 +105,7 @@ hipError_t launch_radii(hipStream_t s, const DevTree& t, float max_radius, float
 hipError_t launch_make_cloud(hipStream_t s, const float* xyzw, const float* radii,
                              const float* colors, int64_t n, float4* xyzr, float4* col);
 hipError_t launch_set_radii(hipStream_t s, const float* radii, int64_t n, float4* xyzr);
+hipError_t launch_max_radius(hipStream_t s, const float4* xyzr, int64_t n, uint32_t* out);
 hipError_t launch_set_colors(hipStream_t s, const float* colors, int64_t n, float4* col);
 hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& v, MarchOut o,
                                int* err, uint32_t* cull, const TreeLevels& lv);

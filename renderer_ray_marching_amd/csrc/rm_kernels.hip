@@ -546,6 +546,47 @@ __device__ __forceinline__ Ray make_ray(F M, float pane, int W, int H, int x, in
     return ray;
 }
 
+// Distance along a ray past which no sphere can be hit. The box of all centres (the root's two
+// child boxes) is dilated by the largest radius, twice MIN_DIST and a float-error margin
+// (10^-5 of the coordinates' scale, about 80 times the rounding of the march point o + d t and of
+// the distance arithmetic); past the ray's exit from that convex box every centre is farther from
+// the march point than its radius + MIN_DIST, so every SDF value there exceeds MIN_DIST and the
+// march can only end as a miss, as it would at the far plane. -INF when the ray misses the box,
+// +INF when the slab arithmetic is indeterminate.
+__device__ __forceinline__ float ray_exit_bound(const DevCloud& c, const Ray& r) {
+    CFloat* f = (CFloat*)c.tree.nodes;
+    CInt* fi = (CInt*)f;
+    float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+        if (fi[14 + s] >= 0)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], f[6 * s + a]);
+                hi[a] = fmaxf(hi[a], f[6 * s + 3 + a]);
+            }
+    float ext = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ext = fmaxf(ext, fmaxf(fabsf(lo[a]), fabsf(hi[a])));
+    const float scale = fmaxf(fmaxf(fabsf(r.ox), fabsf(r.oy)), fabsf(r.oz)) + ext + 1000.0f;
+    const float e = __uint_as_float(*(const __attribute__((address_space(4))) uint32_t*)c.rmax) +
+                    2.0f * kMinDist + 1e-5f * scale;
+    const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+    float tin = -kInf, tout = kInf;
+    bool nan = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float inv = 1.0f / d[a];
+        const float t1 = (lo[a] - e - o[a]) * inv, t2 = (hi[a] + e - o[a]) * inv;
+        nan |= (t1 != t1) | (t2 != t2);
+        tin = fmaxf(tin, fminf(t1, t2));
+        tout = fminf(tout, fmaxf(t1, t2));
+    }
+    if (nan) return kInf;
+    if (tin > tout) return -kInf;
+    return tout + fabsf(tout) * 1e-5f + 1e-3f;
+}
+
 // Signed distance of src/ray_marching.cu:280-290 over the sorted kNN-10: strict minimum of
 // sqrtf(d2) - radius, first entry wins ties; initial best (+INF, 0).
 __device__ __forceinline__ void sdf10(const DevCloud& c, const TopK<kMarchK>& tk, float& s_out,
@@ -655,6 +696,7 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
     int tid = threadIdx.y * 16 + threadIdx.x;
     if (x >= v.W || y >= v.H) return;
     Ray r = make_ray([&](int i) { return v.M[i]; }, v.pane, v.W, v.H, x, y);
+    r.tfar = fminf(r.tfar, ray_exit_bound(c, r));
     int hit;
     float depth;
     march_lane<false, CULL>(c, r, mask, &lstk[tid], 256, maxsp, err, hit, depth);
@@ -737,7 +779,7 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                         dx = r.dx;
                         dy = r.dy;
                         dz = r.dz;
-                        tfar = r.tfar;
+                        tfar = fminf(r.tfar, ray_exit_bound(p.cloud, r));
                         t = r.tnear;
                         steps = 0;
                         pix = y * W + x;
@@ -892,6 +934,21 @@ __global__ void make_cloud_kernel(const float4* __restrict__ xyzw, const float* 
     xyzr[i] = make_float4(p.x, p.y, p.z, radii[i]);
     float4 c = colors[i];
     col[i] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
+// Largest radius (xyzr.w) of a cloud, at least 0, merged into *out by an integer atomic
+// maximum (non-negative floats order like their bit patterns; NaN radii are ignored).
+__global__ __launch_bounds__(256) void max_radius_kernel(const float4* __restrict__ xyzr,
+                                                         int64_t n, uint32_t* out) {
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, xyzr[i].w);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (lane_id() == 0) atomicMax(out, __float_as_uint(m));
 }
 
 __global__ void set_radii_kernel(const float* __restrict__ radii, int64_t n, float4* xyzr) {
@@ -1275,6 +1332,15 @@ hipError_t launch_make_cloud(hipStream_t s, const float* xyzw, const float* radi
     hipLaunchKernelGGL(make_cloud_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s,
                        reinterpret_cast<const float4*>(xyzw), radii,
                        reinterpret_cast<const float4*>(colors), n, xyzr, col);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_radius(hipStream_t s, const float4* xyzr, int64_t n, uint32_t* out) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n == 0) return e;
+    const int blocks = (int)std::min<int64_t>(grid_for(n, 256), 2048);
+    hipLaunchKernelGGL(max_radius_kernel, dim3(blocks), dim3(256), 0, s, xyzr, n, out);
     return hipGetLastError();
 }
 

@@ -448,6 +448,27 @@ def test_culled_batch_equals_plain(rm_ctx, room_culled, ti):
     b.close()
 
 
+def test_culled_render_diff_uses_the_culled_march(rm_ctx, room_culled):
+    """rm_render_diff on a culling cloud selects its hits by the culled march."""
+    import ctypes as C
+    c, views, t, m, rc, plain = room_culled
+    L = _abi.lib()
+    for v, (prgba, phit) in list(zip(views, plain))[:3]:
+        W, H = v.width, v.height
+        img = torch.empty((H, W, 3), device="cuda")
+        dep = torch.empty((H, W), device="cuda")
+        hit = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        pose = (C.c_float * 16)(*[float(x) for x in v.pose_colmajor()])
+        assert L.rm_render_diff(rm_ctx.h, m.h, pose, C.c_float(v.fov), W, H,
+                                C.c_void_p(img.data_ptr()), C.c_void_p(dep.data_ptr()),
+                                C.c_void_p(hit.data_ptr())) == 0
+        rm_ctx.sync()
+        assert torch.equal(hit.reshape(-1), phit)
+        # colour of the hit point, as the plain march wrote it
+        h = phit >= 0
+        assert torch.equal(img.reshape(-1, 3)[h], prgba[h][:, :3])
+
+
 def test_culled_few_and_no_visible_points(rm_ctx):
     """Fewer than 10 points inside the frustum (padding in the warm-started list) and none."""
     fx, w, h = 40.0, 64, 48
