@@ -325,6 +325,59 @@ def test_plain_march_tiny_cloud_and_misses(rm_ctx):
     assert (ohit == -1).any() and (ohit >= 0).any()
 
 
+def test_early_miss_edge_cases(rm_ctx):
+    """Rays are ended as misses past their exit from the cloud's dilated box. Check the plain
+    and batched marches against the oracle (which marches every step) on cameras inside the box,
+    outside looking at it, outside looking away, grazing a face, with a huge radius in the cloud,
+    after the radii change (the bound's largest radius is recomputed) and after the points move
+    (the box is refitted)."""
+    rng = np.random.default_rng(17)
+    n = 3000
+    xyz = np.concatenate([rng.uniform([-1, -1, -1], [1, 1, -0.98], (n // 2, 3)),     # a floor
+                          rng.uniform([-1, 0.98, -1], [1, 1, 1], (n - n // 2, 3))])  # a wall
+    xyz = xyz.astype(np.float32)
+    cloud = scenes.Cloud(xyz, rng.integers(0, 256, (n, 3)).astype(np.uint8))
+    t, m, rc = _marcher(rm_ctx, cloud, 0.0)
+    W, H, fx = 48, 36, 30.0
+    views = [scenes.View(scenes.look_at(e, c), W, H, fx) for e, c in [
+        ((0.0, 0.0, 0.0), (0.0, 1.0, -0.5)),        # inside the box
+        ((0.0, -4.0, 0.5), (0.0, 0.0, -0.5)),       # outside, looking at it
+        ((0.0, -4.0, 0.5), (0.0, -8.0, 0.5)),       # outside, looking away
+        ((-3.0, -0.5, -0.97), (3.0, -0.5, -0.97)),  # grazing the floor
+    ]]
+    pts = cloud.xyzw()
+
+    def check(radii_np):
+        rt = ob.RefTree(pts, 64)
+        b = api.Batch(m, views, tune=TUNES[1])
+        b.launch()
+        rm_ctx.sync()
+        off = 0
+        for v in views:
+            rgba, hit = m.render_to_texture(v.pose_colmajor(), v.fov, W, H)
+            rm_ctx.sync()
+            orgba, ohit, _ = rt.render(radii_np, cloud.colors4(), v.pose_colmajor(), v.fov, W, H)
+            assert (hit.cpu().numpy().ravel() == ohit).all()
+            assert (rgba.cpu().numpy().reshape(-1, 4) == orgba).all()
+            assert (b.hit[off:off + W * H].cpu().numpy() == ohit).all()
+            off += W * H
+        b.close()
+        return off
+
+    check(rc.cpu().numpy())
+    big = rc.clone()
+    big[7] = 10.0                                    # one huge sphere
+    m.set_radii(big)
+    check(big.cpu().numpy())
+    small = torch.full_like(rc, 0.02)
+    m.set_radii(small)                               # the largest radius shrinks again
+    check(small.cpu().numpy())
+    moved = xyz + np.float32(0.5)                    # the box moves with the points
+    m.refit(_dev(moved))
+    pts = np.concatenate([moved, np.ones((n, 1), np.float32)], 1)
+    check(small.cpu().numpy())
+
+
 # ------------------------------------------------------------------------------------------
 # batched march == plain march, for every traversal mode / tile / warm start
 # ------------------------------------------------------------------------------------------
