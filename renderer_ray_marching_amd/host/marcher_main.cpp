@@ -4,8 +4,9 @@
 // clamped after loading), matrices JSON ("train" then "val", sorted keys), per view
 // <output>/<parent>/<file> colour PNG, "_color" -> "_depth" depth PNG plus <depth>.npy,
 // skip-existing, advisory .<file>.lock files. The GPU work goes through librm_hip.so (include/rm.h):
-// one tree build, one fused radii kernel, and the march, either one view per launch (default, as
-// the reference) or -b N views per batched launch.
+// one tree build, one fused radii kernel, and the march: one view per launch by default, as the
+// reference, through the batched kernel (packet traversal, warm start; the images are identical
+// to the plain kernel's), -b N views per batched launch, or -b 0 for the plain per-view kernel.
 //
 // Deliberate differences (the reference's behaviour is undefined or fatal there):
 //   * errors end the program with a message and exit status 1 instead of an uncaught exception;
@@ -54,6 +55,7 @@ struct Args {
     float max_radius = 0.1f;
     bool headless = false, precompute = false, ignore_existing = false;
     int batch = 1;
+    bool plain = false;   // -b 0: the plain per-view kernel (rm_render)
 };
 
 // CLI11 exit codes for the errors this parser reports.
@@ -75,7 +77,8 @@ const char* kHelp =
     "  -d,--headless               Run headlessly without a window\n"
     "  -p,--precompute-radii       Precompute radii even if already precomputed.\n"
     "  -i,--ignore_existing        Ignore existing renders and forcefully rewrite them.\n"
-    "  -b,--batch INT              Views marched per batched GPU launch (default 1: one launch per view).\n";
+    "  -b,--batch INT              Views marched per batched GPU launch (default 1: one launch per view;\n"
+    "                              0: the plain per-view kernel).\n";
 
 struct Opt {
     char short_name;
@@ -201,9 +204,13 @@ int parse_args(int argc, char** argv, Args& a) {
         std::cerr << endl << "Run with --help for more information." << endl;
         return kExtrasError;
     }
-    if (a.batch < 1) {
-        std::cerr << "-b,--batch must be at least 1" << endl;
+    if (a.batch < 0) {
+        std::cerr << "-b,--batch must be at least 0" << endl;
         return kConversionError;
+    }
+    if (a.batch == 0) {
+        a.plain = true;
+        a.batch = 1;
     }
     return -1;
 }
@@ -356,7 +363,7 @@ private:
             rgba_bytes_ = bytes;
         }
         const auto start = std::chrono::high_resolution_clock::now();
-        if (group.size() == 1) {
+        if (a_.plain) {
             const rmh::ViewSpec& s = group[0].spec;
             gpu_.check(rm_render(gpu_.ctx, cloud_, s.pose, s.fov_radians, s.width, s.height,
                                  rgba_.as<float>(), nullptr), "rm_render");

@@ -126,6 +126,13 @@ def test_cli_render_cache_skip_and_batch(scene, tmp_path):
     assert "batch of 3 views" in r.stdout
     for k in scene["views"]:
         _assert_view(out, k, exp[k])
+    # and with the plain per-view kernel (-b 0): the same files
+    for k in scene["views"]:
+        os.remove(_outputs(out, k)[0])
+    r = _run(args + ["-i", "-b", "0"], d)
+    assert r.returncode == 0, r.stderr
+    for k in scene["views"]:
+        _assert_view(out, k, exp[k])
 
 
 def test_cli_lock_held_by_another_process(scene):

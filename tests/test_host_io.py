@@ -386,6 +386,7 @@ def test_cli_arguments(tmp_path):
     assert _run(["positional"]).returncode == 109
     assert _run(["-s", "abc"]).returncode == 104
     assert _run(["-r", "1.5x"]).returncode == 104
+    assert _run(["-b", "-1"]).returncode == 104                # -b 0 (plain kernel) and up only
     assert _run(["-f"]).returncode == 114
     assert _run(["-f", "a", "--file", "b"]).returncode == 114
     r = _run(["-d", "-f", str(tmp_path / "missing.ply")])
