@@ -624,3 +624,13 @@ def test_full_size_room_10m_256_views(rm_ctx):
     cloud, views, max_r = scenes.workload_room_10m()
     n = _full_size_check(rm_ctx, cloud, views, max_r, [0, 37, 101, 190, 255], 16)
     assert n > 100_000
+
+
+@pytest.mark.slow
+def test_full_size_scale_50m_view_subset(rm_ctx):
+    """The largest configuration's cloud (50M points, radii against the oracle on all of them)
+    with every 64th of its 1024 views at 1920x1080 (the batch output of all 1024 views would be
+    42 GB on the host)."""
+    cloud, views, max_r = scenes.workload_scale_50m()
+    n = _full_size_check(rm_ctx, cloud, views[::64], max_r, [0, 5, 10, 15], 64)
+    assert n > 100_000
