@@ -112,7 +112,8 @@ int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* cloud, const float* colors);
  * Applies to rm_render, rm_render_diff and to batches created after the call (a batch keeps the
  * setting it was created with). Enabling allocates one view's culling data on the cloud, which
  * single-view renders share: culled rm_render / rm_render_diff calls on one cloud must be
- * ordered (same stream), while each batch has its own. */
+ * ordered (same stream), while each batch has its own (a culled batch holds at most 65535
+ * views: its per-view node pass uses the grid's second dimension). */
 int rm_cloud_set_culling(rm_ctx* ctx, rm_cloud* cloud, int enable);
 /* Move the points of a cloud (xyz: n x 3 floats, original order) and refit the boxes of the
  * tree it was created from, keeping the topology. Search stays exact because every bound is
