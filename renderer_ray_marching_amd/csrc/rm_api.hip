@@ -731,7 +731,8 @@ int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
             return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad view " + std::to_string(i));
     }
     if (cloud->cull && nviews > 65535)
-        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_batch_create: a culled batch holds at most 65535 views");
+        return fail(ctx, RM_ERR_UNSUPPORTED,
+                    "rm_batch_create: a culled batch holds at most 65535 views");
     rm_tune t = tune ? *tune : default_tune();
     if (!tune_valid(t)) return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad tune");
     if (t.warm && cloud->n < rm::kMarchK) t.warm = 0;
