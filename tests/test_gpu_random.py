@@ -95,3 +95,44 @@ def test_random_parity(rm_ctx, block):
             off += npx
         b.close()
         m.close()
+
+
+def test_random_knn_and_radius_search(rm_ctx):
+    """100 seeded cases: kNN (k 1..32) and radius search (capped and counting) against brute
+    force, on random clouds up to 20k points with queries on points, near them and far away."""
+    import fp32
+    for case in range(100):
+        rng = np.random.default_rng(5000 + case)
+        n = int(rng.choice([1, 2, 9, 33, 400, 3000, 20000]))
+        xyz = rng.normal(0, rng.uniform(0.05, 2.0), (n, 3)).astype(np.float32)
+        if n > 4 and rng.integers(0, 2):
+            xyz[: n // 3] = xyz[n // 3: 2 * (n // 3)]
+        p = np.concatenate([xyz, np.ones((n, 1), np.float32)], 1)
+        m = int(rng.integers(1, 200))
+        q = p[rng.integers(0, n, m)].copy()
+        q[: m // 3, :3] += rng.normal(0, 0.05, (m // 3, 3)).astype(np.float32)
+        q[m // 3: m // 2, :3] = rng.uniform(-20, 20, (m // 2 - m // 3, 3)).astype(np.float32)
+        tree = api.KDTreeFlann(rm_ctx, leaf_max=int(rng.choice([0, 1, 4, 16, 64])))
+        tree.Build(torch.from_numpy(p).cuda())
+        qd = torch.from_numpy(np.ascontiguousarray(q)).cuda()
+        k = int(rng.integers(1, 33))
+        _, idx, d2 = tree.SearchKNN(qd, k)
+        rm_ctx.sync()
+        oi, od = ob.knn_brute(p, q, k)
+        assert (idx.cpu().numpy() == oi).all() and (d2.cpu().numpy() == od).all(), case
+        radius = float(rng.uniform(0.0, 1.0))
+        max_nn = int(rng.integers(0, 33))
+        total, ridx, rd2, cnt = tree.SearchRadius(qd, radius, max_nn)
+        rm_ctx.sync()
+        r2 = np.float32(radius) * np.float32(radius)
+        dall = fp32.dist2(p[None, :, :], q[:, None, :])
+        inside = (dall < r2).sum(1)
+        if max_nn == 0:
+            assert (cnt.cpu().numpy() == inside).all(), case
+        else:
+            bi, bd = ob.knn_brute(p, q, max_nn)
+            ok = bd < r2
+            assert (ridx.cpu().numpy() == np.where(ok, bi, -1)).all(), case
+            assert (rd2.cpu().numpy() == np.where(ok, bd, np.float32(np.inf))).all(), case
+            assert (cnt.cpu().numpy() == np.minimum(inside, max_nn)).all(), case
+        assert total == int(cnt.cpu().numpy().sum()), case
