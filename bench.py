@@ -188,7 +188,7 @@ def main():
         if alg and alg["points"] == cloud.n and alg["rays"] == rays and not a.cull:
             b_launch = alg["march_bytes_per_ray"] * rays
             ach = b_launch / (avg_kernel_ms * 1e-3) / 1e9
-            pmc = measured_traffic(a.workload, tune)
+            pmc = measured_traffic(a.workload)
             roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                         "traffic": pmc.get("TRAFFIC_BYTES_PER_LAUNCH"),
@@ -201,6 +201,7 @@ def main():
                                              if "TRAFFIC_BYTES_PER_LAUNCH" in pmc else None),
                         "l2_hit_rate": pmc.get("L2_HIT_RATE"),
                         "valu_busy": pmc.get("VALU_BUSY"),
+                        "pmc_tune": pmc.get("PMC_TUNE"),
                         "kernel_ms": round(avg_kernel_ms, 3)}
             rb = alg["radii_bytes_total"]
             rach = rb / (radii_ms * 1e-3) / 1e9
@@ -262,11 +263,12 @@ def kernel_source_sha256():
     return hashlib.sha256(open(src, "rb").read()).hexdigest()
 
 
-def measured_traffic(workload, tune):
-    """HBM bytes per launch of the march kernel from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE passes, calibrated with a 1 GiB copy), used only when that
-    summary was taken on the same workload, with the same launch tune and from the same kernel
-    source (SHA-256 of rm_kernels.hip)."""
+def measured_traffic(workload):
+    """HBM bytes per launch of the march and radii kernels from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes, calibrated with a 1 GiB copy), used only when that
+    summary was taken on the same workload and from the same kernel source (SHA-256 of
+    rm_kernels.hip). The launch tune it was taken with is returned as PMC_TUNE: the autotuner may
+    pick an equivalent block shape in another run, and the report names the profiled one."""
     path = os.path.join(ROOT, "profiles", "march_batch_room10m_pmc.txt")
     if not os.path.exists(path):
         return {}
@@ -276,12 +278,13 @@ def measured_traffic(workload, tune):
             k, v = line.split(":", 1)
             rec[k.strip()] = v.strip()
     try:
-        if rec.get("WORKLOAD") != workload or json.loads(rec.get("TUNE", "null")) != tune:
+        if rec.get("WORKLOAD") != workload:
             return {}
         if rec.get("KERNEL_SRC_SHA256") != kernel_source_sha256():
             return {}
         out = {"TRAFFIC_BYTES_PER_LAUNCH": int(rec["TRAFFIC_BYTES_PER_LAUNCH"]),
-               "L2_HIT_RATE": float(rec["L2_HIT_RATE"])}
+               "L2_HIT_RATE": float(rec["L2_HIT_RATE"]),
+               "PMC_TUNE": json.loads(rec["TUNE"])}
         if "VALU_BUSY" in rec:
             out["VALU_BUSY"] = float(rec["VALU_BUSY"])
         if "RADII_TRAFFIC_BYTES" in rec:
