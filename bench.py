@@ -97,6 +97,7 @@ def main():
     radii_ms = float(np.median(radii_ms))
     log(f"radii kernel: {radii_ms:.3f} ms")
 
+    search = search_leg(api, ctx, tree, p, torch)
     marcher = api.PointcloudRayMarcher.get_instance(p, col, rc, tree)
     if a.cull:
         marcher.set_culling(True)
@@ -248,6 +249,7 @@ def main():
         "tune": tune,
         "radii_ms": round(radii_ms, 4),
         "radii_roofline": radii_roof,
+        "search": search,
         "roofline": roofline,
         "diff": diff,
         "cpu_baseline": cpu,
@@ -261,6 +263,41 @@ def kernel_source_sha256():
     import hashlib
     src = os.path.join(ROOT, "renderer_ray_marching_amd", "csrc", "rm_kernels.hip")
     return hashlib.sha256(open(src, "rb").read()).hexdigest()
+
+
+def search_leg(api, ctx, tree, p, torch):
+    """Batched queries against the workload's tree (the SearchKNN / SearchRadius API): 1M
+    queries, points of the cloud moved by N(0, 1 cm); device time per launch (median of 5)."""
+    g = torch.Generator(device="cuda").manual_seed(7)
+    m = 1_000_000
+    sel = torch.randint(0, p.shape[0], (m,), device="cuda", generator=g)
+    q = p[sel].clone()
+    q[:, :3] += 0.01 * torch.randn((m, 3), device="cuda", generator=g)
+    q = q.contiguous()
+    out = {"queries": m}
+
+    def timed(fn):
+        fn()
+        ctx.sync()
+        ms = []
+        for _ in range(5):
+            fn()
+            ms.append(ctx.last_kernel_ms())
+        ctx.sync()
+        return float(np.median(ms))
+
+    ms = timed(lambda: tree.SearchKNN(q, 10))
+    out["knn10_ms"] = round(ms, 3)
+    out["knn10_mqueries_s"] = round(m / ms / 1e3, 1)
+    ms = timed(lambda: tree.SearchRadius(q, 0.05, 16))
+    out["radius_r005_nn16_ms"] = round(ms, 3)
+    out["radius_mqueries_s"] = round(m / ms / 1e3, 1)
+    ms = timed(lambda: tree.SearchRadius(q, 0.05, 0))
+    out["radius_count_r005_ms"] = round(ms, 3)
+    out["radius_count_mqueries_s"] = round(m / ms / 1e3, 1)
+    log(f"search: kNN-10 {out['knn10_mqueries_s']} Mq/s, radius {out['radius_mqueries_s']} Mq/s, "
+        f"count {out['radius_count_mqueries_s']} Mq/s")
+    return out
 
 
 def measured_traffic(workload):

@@ -45,8 +45,9 @@ int rm_ctx_destroy(rm_ctx* ctx);
 int rm_ctx_set_stream(rm_ctx* ctx, void* hip_stream);
 int rm_ctx_synchronize(rm_ctx* ctx);
 const char* rm_last_error(const rm_ctx* ctx);
-/* Device time (ms) of the last kernel launched by rm_radii / rm_render / rm_batch_launch,
- * measured with HIP events on the stream the kernel ran on (0 if events are off). */
+/* Device time (ms) of the last call of rm_knn / rm_radius_search / rm_radii / rm_render /
+ * rm_render_diff / rm_batch_launch / rm_batch_render_diff, measured with HIP events on the
+ * stream its kernels ran on (0 if events are off). */
 int rm_ctx_enable_timing(rm_ctx* ctx, int on);
 float rm_ctx_last_kernel_ms(const rm_ctx* ctx);
 

@@ -463,7 +463,9 @@ int rm_knn(rm_ctx* ctx, const rm_tree* tree, const float* q, int64_t m, int k, i
     if (m < 0 || (m > 0 && (!q || !idx || !d2)))
         return fail(ctx, RM_ERR_INVALID, "rm_knn: bad buffers");
     if (int r = set_device(ctx)) return r;
+    timing_begin(ctx);
     RM_HIP(ctx, rm::launch_knn(ctx->stream, dev_tree(tree), q, m, k, idx, d2, ctx->d_err));
+    timing_end(ctx);
     return RM_OK;
 }
 
@@ -480,8 +482,10 @@ int rm_radius_search(rm_ctx* ctx, const rm_tree* tree, const float* q, int64_t m
         return fail(ctx, RM_ERR_INVALID, "rm_radius_search: bad buffers");
     if (int r = set_device(ctx)) return r;
     float r2 = radius * radius;  // float(radius * radius), src/kdtree/kdtree_flann.cu:98
+    timing_begin(ctx);
     RM_HIP(ctx, rm::launch_radius(ctx->stream, dev_tree(tree), q, m, r2, max_nn, idx, d2, counts,
                                   ctx->d_err));
+    timing_end(ctx);
     return RM_OK;
 }
 
