@@ -160,7 +160,8 @@ int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
                     const rm_tune* tune, float* rgba, int32_t* hit, rm_batch** out);
 /* Time candidate rm_tune settings on every sample_stride-th view (whole views, so the sample
  * keeps each view's screen locality and the mix of view sizes) and keep the
- * fastest (re-captures the graph). chosen may be NULL. */
+ * fastest (re-captures the graph). chosen may be NULL. On an error the batch keeps the tune,
+ * layout and graph it had before the call. */
 int rm_batch_autotune(rm_batch* batch, int sample_stride, rm_tune* chosen);
 int rm_batch_get_tune(const rm_batch* batch, rm_tune* tune);
 int rm_batch_launch(rm_batch* batch);

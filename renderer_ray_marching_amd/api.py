@@ -25,6 +25,20 @@ def _ptr(t: torch.Tensor | None):
     return C.c_void_p(t.data_ptr())
 
 
+def _f32(t: torch.Tensor, name: str, numel: int | None = None, cols: int | None = None):
+    """Contiguous float32 CUDA view of t, checked before its pointer reaches the library (which
+    reads exactly numel values, or rows of `cols` values)."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name}: expected a CUDA tensor")
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name}: expected float32, got {t.dtype}")
+    if cols is not None and t.numel() % cols:
+        raise ValueError(f"{name}: {t.numel()} values are not rows of {cols}")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: expected {numel} values, got {t.numel()}")
+    return t.contiguous()
+
+
 class Context:
     """One rm_ctx: a device and a stream (the current torch stream, so torch ops and library
     kernels are ordered)."""
@@ -100,7 +114,7 @@ class KDTreeFlann:
         self.n = 0
 
     def Build(self, pcd: torch.Tensor) -> bool:
-        pcd = pcd.reshape(-1, 4).contiguous()
+        pcd = _f32(pcd, "Build: points", cols=4).reshape(-1, 4)
         self._free()
         h = C.c_void_p()
         check(lib().rm_tree_build(self.ctx.h, _ptr(pcd), pcd.shape[0], self.leaf_max,
@@ -123,7 +137,7 @@ class KDTreeFlann:
 
     def SearchKNN(self, query: torch.Tensor, knn: int):
         """Returns (k * M, idx (M, k) int32, d2 (M, k) float32), as kdtree_flann.cu:65-80."""
-        q = query.reshape(-1, 4).contiguous()
+        q = _f32(query, "SearchKNN: query", cols=4).reshape(-1, 4)
         m = q.shape[0]
         idx = torch.empty((m, knn), dtype=torch.int32, device=q.device)
         d2 = torch.empty((m, knn), dtype=torch.float32, device=q.device)
@@ -138,7 +152,7 @@ class KDTreeFlann:
         max_nn == 0 only the uncapped per-query counts are computed (idx and d2 are (M, 0)),
         the counting mode of radiusSearchGpu (kdtree_cuda_3d_index.cu:436-452), whose return
         value is their sum."""
-        q = query.reshape(-1, 4).contiguous()
+        q = _f32(query, "SearchRadius: query", cols=4).reshape(-1, 4)
         m = q.shape[0]
         idx = torch.empty((m, max_nn), dtype=torch.int32, device=q.device)
         d2 = torch.empty((m, max_nn), dtype=torch.float32, device=q.device)
@@ -150,8 +164,9 @@ class KDTreeFlann:
 
     def radii(self, max_radius: float = 0.1):
         """Per-point radii of main.cu:99-149 (unclamped, clamped), original order."""
-        ru = torch.empty((self.n,), dtype=torch.float32, device="cuda")
-        rc = torch.empty((self.n,), dtype=torch.float32, device="cuda")
+        dev = torch.device("cuda", self.ctx.device)
+        ru = torch.empty((self.n,), dtype=torch.float32, device=dev)
+        rc = torch.empty((self.n,), dtype=torch.float32, device=dev)
         check(lib().rm_radii(self.ctx.h, self.h, float(max_radius), _ptr(ru), _ptr(rc)),
               self.ctx.h)
         return ru, rc
@@ -184,10 +199,10 @@ class PointcloudRayMarcher:
                  tree: KDTreeFlann):
         self.ctx = tree.ctx
         self.tree = tree
-        v = vertices.reshape(-1, 4).contiguous()
-        c = colors.reshape(-1, 4).contiguous()
-        r = radii.reshape(-1).contiguous()
-        self.n = int(v.shape[0])
+        v = _f32(vertices, "vertices", numel=4 * tree.n)
+        c = _f32(colors, "colors", numel=4 * tree.n)
+        r = _f32(radii, "radii", numel=tree.n)
+        self.n = tree.n
         h = C.c_void_p()
         check(lib().rm_cloud_create(self.ctx.h, tree.h, _ptr(v), _ptr(c), _ptr(r), self.n,
                                     C.byref(h)), self.ctx.h)
@@ -202,25 +217,32 @@ class PointcloudRayMarcher:
                           hit: torch.Tensor | None = None):
         """Render one view into an (H, W, 4) float32 'texture' (row 0 = bottom, a = depth)
         and an (H, W) int32 hit map; returns (rgba, hit)."""
+        dev = torch.device("cuda", self.ctx.device)
         if rgba is None:
-            rgba = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+            rgba = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+        else:
+            rgba = _f32(rgba, "rgba", numel=4 * width * height)
         if hit is None:
-            hit = torch.empty((height, width), dtype=torch.int32, device="cuda")
+            hit = torch.empty((height, width), dtype=torch.int32, device=dev)
+        elif hit.dtype != torch.int32 or hit.numel() != width * height or not hit.is_cuda:
+            raise ValueError("hit: expected an int32 CUDA tensor of width * height values")
         pose = (C.c_float * 16)(*np.asarray(camera_pose_colmajor, dtype=np.float32).ravel())
         check(lib().rm_render(self.ctx.h, self.h, pose, float(fov_radians), width, height,
                               _ptr(rgba), _ptr(hit)), self.ctx.h)
         return rgba, hit
 
     def set_radii(self, radii: torch.Tensor):
-        check(lib().rm_cloud_set_radii(self.ctx.h, self.h, _ptr(radii.contiguous())),
-              self.ctx.h)
+        check(lib().rm_cloud_set_radii(self.ctx.h, self.h,
+                                       _ptr(_f32(radii, "set_radii", numel=self.n))), self.ctx.h)
 
     def set_colors(self, colors: torch.Tensor):
-        check(lib().rm_cloud_set_colors(self.ctx.h, self.h, _ptr(colors.contiguous())),
+        check(lib().rm_cloud_set_colors(self.ctx.h, self.h,
+                                        _ptr(_f32(colors, "set_colors", numel=4 * self.n))),
               self.ctx.h)
 
     def refit(self, xyz: torch.Tensor):
-        check(lib().rm_cloud_refit(self.ctx.h, self.h, _ptr(xyz.contiguous())), self.ctx.h)
+        check(lib().rm_cloud_refit(self.ctx.h, self.h, _ptr(_f32(xyz, "refit", numel=3 * self.n))),
+              self.ctx.h)
 
     def set_culling(self, enable: bool):
         """March each view over the points inside its frustum (is_inside of
@@ -263,8 +285,9 @@ class Batch:
         self.m = marcher
         self.ctx = marcher.ctx
         self.views_c, self.total = make_views(views)
-        self.rgba = torch.empty((self.total, 4), dtype=torch.float32, device="cuda")
-        self.hit = (torch.empty((self.total,), dtype=torch.int32, device="cuda")
+        dev = torch.device("cuda", self.ctx.device)
+        self.rgba = torch.empty((self.total, 4), dtype=torch.float32, device=dev)
+        self.hit = (torch.empty((self.total,), dtype=torch.int32, device=dev)
                     if with_hit else None)
         t = None
         if tune is not None:

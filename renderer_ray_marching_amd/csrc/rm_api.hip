@@ -197,6 +197,11 @@ int check_tree(rm_ctx* ctx, const rm_tree* t) {
     return RM_OK;
 }
 
+int check_cloud(rm_ctx* ctx, const rm_cloud* c) {
+    if (!c) return fail(ctx, RM_ERR_INVALID, "null cloud");
+    return check_tree(ctx, c->tree);
+}
+
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
@@ -545,6 +550,7 @@ int rm_cloud_destroy(rm_cloud* c) {
 
 int rm_cloud_set_radii(rm_ctx* ctx, rm_cloud* c, const float* radii) {
     if (!ctx || !c || !radii) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_radii: null input");
+    if (int r = check_cloud(ctx, c)) return r;
     if (int r = set_device(ctx)) return r;
     RM_HIP(ctx, rm::launch_set_radii(ctx->stream, radii, c->n, c->xyzr));
     RM_HIP(ctx, rm::launch_max_radius(ctx->stream, c->xyzr, c->n, c->rmax));
@@ -553,6 +559,7 @@ int rm_cloud_set_radii(rm_ctx* ctx, rm_cloud* c, const float* radii) {
 
 int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* c, const float* colors) {
     if (!ctx || !c || !colors) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_colors: null input");
+    if (int r = check_cloud(ctx, c)) return r;
     if (int r = set_device(ctx)) return r;
     RM_HIP(ctx, rm::launch_set_colors(ctx->stream, colors, c->n, c->colors));
     return RM_OK;
@@ -562,6 +569,7 @@ int rm_cloud_set_culling(rm_ctx* ctx, rm_cloud* c, int enable) {
     if (!ctx || !c) return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_culling: null input");
     if (enable != 0 && enable != 1)
         return fail(ctx, RM_ERR_INVALID, "rm_cloud_set_culling: enable must be 0 or 1");
+    if (int r = check_cloud(ctx, c)) return r;
     if (enable && !c->mask) {
         if (int r = set_device(ctx)) return r;
         const size_t bytes =
@@ -579,6 +587,7 @@ int rm_cloud_set_culling(rm_ctx* ctx, rm_cloud* c, int enable) {
 
 int rm_cloud_refit(rm_ctx* ctx, rm_cloud* c, const float* xyz) {
     if (!ctx || !c || !xyz) return fail(ctx, RM_ERR_INVALID, "rm_cloud_refit: null input");
+    if (int r = check_cloud(ctx, c)) return r;
     if (int r = set_device(ctx)) return r;
     rm_tree* t = c->tree;
     RM_HIP(ctx, rm::launch_refit(ctx->stream, t->nodes, t->pts, c->xyzr, xyz, t->n,
@@ -596,6 +605,7 @@ int rm_render(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov, int 
     if (!c || !pose || !rgba) return fail(ctx, RM_ERR_INVALID, "rm_render: null input");
     if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render: empty image");
     if (!(fov > 0.0f && fov < 3.14159f)) return fail(ctx, RM_ERR_INVALID, "rm_render: bad fov");
+    if (int r = check_cloud(ctx, c)) return r;
     if (int r = set_device(ctx)) return r;
     DevView v = dev_view(pose, fov, W, H, 0, 0, 0);
     timing_begin(ctx);
@@ -734,6 +744,7 @@ int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
             !(v.fov_radians > 0.0f && v.fov_radians < 3.14159f))
             return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad view " + std::to_string(i));
     }
+    if (int r = check_cloud(ctx, cloud)) return r;
     if (cloud->cull && nviews > 65535)
         return fail(ctx, RM_ERR_UNSUPPORTED,
                     "rm_batch_create: a culled batch holds at most 65535 views");
@@ -812,9 +823,12 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
     rm_ctx* ctx = b->ctx;
     if (sample_stride < 1) sample_stride = 1;
     if (int r = set_device(ctx)) return r;
-    hipEvent_t e0, e1;
-    RM_HIP(ctx, hipEventCreate(&e0));
-    RM_HIP(ctx, hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        const hipError_t e = hipGetLastError();
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(ctx, e, "rm_batch_autotune: hipEventCreate");
+    }
     auto measure = [&](const rm_tune& t, float& ms) -> int {
         b->tune = t;
         if (int r = batch_layout(b, sample_stride)) return r;
@@ -831,6 +845,7 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         ms = best;
         return RM_OK;
     };
+    const rm_tune incumbent = b->tune;
     rm_tune best = b->tune;
     float best_ms = 1e30f;
     const char* env = std::getenv("RM_TUNE_VERBOSE");
@@ -893,7 +908,15 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (r != RM_OK) return r;
+    if (r != RM_OK) {
+        // the view table holds a sampled layout now: put the incumbent tune, its layout and
+        // its graph back so that the batch stays launchable, and report the first error
+        const std::string msg = ctx->err;
+        b->tune = incumbent;
+        if (batch_layout(b) == RM_OK) (void)batch_capture(b);
+        ctx->err = msg;
+        return r;
+    }
     b->tune = best;
     if (int rr = batch_layout(b)) return rr;
     if (int rr = batch_capture(b)) return rr;
@@ -934,6 +957,7 @@ int rm_render_diff(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov,
         return fail(ctx, RM_ERR_INVALID, "rm_render_diff: null input");
     if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render_diff: empty image");
     if (!(fov > 0.0f && fov < 3.14159f)) return fail(ctx, RM_ERR_INVALID, "rm_render_diff: bad fov");
+    if (int r = check_cloud(ctx, c)) return r;
     if (int r = set_device(ctx)) return r;
     int64_t npix = (int64_t)W * H;
     if (int r = ensure_ws(ctx, align_up(sizeof(float4) * (size_t)npix))) return r;
@@ -955,6 +979,9 @@ int rm_render_adjoint(rm_ctx* ctx, const rm_cloud* c, const float* pose, float f
     if (!c || !pose || !hit || !grad_rgb || !grad_depth || !g_xyz || !g_r || !g_rgb)
         return fail(ctx, RM_ERR_INVALID, "rm_render_adjoint: null input");
     if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render_adjoint: empty image");
+    if (!(fov > 0.0f && fov < 3.14159f))
+        return fail(ctx, RM_ERR_INVALID, "rm_render_adjoint: bad fov");
+    if (int r = check_cloud(ctx, c)) return r;
     if (int r = set_device(ctx)) return r;
     int64_t npix = (int64_t)W * H;
     if (npix > INT32_MAX) return fail(ctx, RM_ERR_UNSUPPORTED, "rm_render_adjoint: image too large");

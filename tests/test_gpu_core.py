@@ -325,6 +325,34 @@ def test_plain_march_tiny_cloud_and_misses(rm_ctx):
     assert (ohit == -1).any() and (ohit >= 0).any()
 
 
+def test_mirror_rejects_bad_tensors(rm_ctx):
+    """The Python mirror checks dtype and length before a pointer reaches the library, which
+    reads exactly n values: a short or float64 tensor is an error, not an out-of-bounds read."""
+    cloud = scenes.box_scene(2_000, seed=5)
+    t, m, rc = _marcher(rm_ctx, cloud, 0.1)
+    n = cloud.n
+    with pytest.raises(ValueError):
+        m.set_radii(rc[:-1])
+    with pytest.raises(TypeError):
+        m.set_radii(rc.double())
+    with pytest.raises(ValueError):
+        m.set_colors(torch.ones((n - 1, 4), device="cuda"))
+    with pytest.raises(ValueError):
+        m.refit(torch.zeros((n, 4), device="cuda"))
+    with pytest.raises(TypeError):
+        api.KDTreeFlann(rm_ctx).Build(_dev(cloud.xyzw(), torch.float64))
+    with pytest.raises(ValueError):
+        t.SearchKNN(torch.zeros((5, 3), device="cuda"), 2)
+    with pytest.raises(ValueError):
+        api.PointcloudRayMarcher(_dev(cloud.xyzw()), _dev(cloud.colors4()[:-1]), rc, t)
+    with pytest.raises(ValueError):
+        m.render_to_texture(np.eye(4), 1.0, 32, 24, rgba=torch.empty((24, 31, 4), device="cuda"))
+    # still usable afterwards
+    rgba, hit = m.render_to_texture(np.eye(4), 1.0, 32, 24)
+    rm_ctx.sync()
+    assert hit.shape == (24, 32)
+
+
 def test_early_miss_edge_cases(rm_ctx):
     """Rays are ended as misses past their exit from the cloud's dilated box. Check the plain
     and batched marches against the oracle (which marches every step) on cameras inside the box,
