@@ -256,6 +256,42 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
     assert np.median(err / np.maximum(1.0, np.abs(g[pick]))) < 2e-3
 
 
+def test_batch_render_diff_mixed_sizes(rm_ctx, small_scene):
+    """rm_batch_render_diff over views of different (odd) sizes writes each view's rgb / depth
+    at that view's offset, equal to rm_render_diff of the view alone on every pixel."""
+    import ctypes as C
+    from renderer_ray_marching_amd import _abi, api
+    cloud, rc, views, R, xyz, r, rgb = small_scene
+    vs = [scenes.box_scene_view(cloud, w, h, f, distance=5.0, seed=s)
+          for (w, h, f, s) in ((96, 72, 80.0, 4), (33, 17, 30.0, 5), (64, 130, 70.0, 6))]
+    b = api.Batch(R.marcher, vs)
+    total = sum(v.width * v.height for v in vs)
+    rgb_b = torch.full((total, 3), -1.0, device="cuda")
+    dep_b = torch.full((total,), -1.0, device="cuda")
+    L = _abi.lib()
+    assert L.rm_batch_render_diff(b.h, C.c_void_p(rgb_b.data_ptr()),
+                                  C.c_void_p(dep_b.data_ptr())) == 0
+    rm_ctx.sync()
+    off = 0
+    for view in vs:
+        W, H = view.width, view.height
+        im1 = torch.empty((H, W, 3), device="cuda")
+        d1 = torch.empty((H, W), device="cuda")
+        h1 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        pose = (C.c_float * 16)(*[float(x) for x in view.pose_colmajor()])
+        assert L.rm_render_diff(rm_ctx.h, R.marcher.h, pose, C.c_float(view.fov), W, H,
+                                C.c_void_p(im1.data_ptr()), C.c_void_p(d1.data_ptr()),
+                                C.c_void_p(h1.data_ptr())) == 0
+        rm_ctx.sync()
+        n = W * H
+        assert (h1 >= 0).any()
+        assert torch.equal(b.hit[off:off + n], h1.reshape(-1))
+        assert torch.equal(rgb_b[off:off + n], im1.reshape(-1, 3))
+        assert torch.equal(dep_b[off:off + n], d1.reshape(-1))
+        off += n
+    b.close()
+
+
 def test_batched_forward_equals_per_view(rm_ctx, small_scene):
     """The op's forward (one batched march over all views, rm_batch_render_diff) equals
     rm_render_diff called view by view, on every pixel."""
