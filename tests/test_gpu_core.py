@@ -191,6 +191,33 @@ def test_radius_count_mode_vs_brute(rm_ctx, radius):
     assert (cap == np.minimum(exp, 32)).all()
 
 
+def test_radius_search_edge_radii(rm_ctx):
+    """Radius 0 finds nothing, not even a query's own duplicate (d2 < 0 never holds); a radius
+    whose float square overflows to +inf admits every point, so the capped mode returns the
+    max_nn nearest and counting returns n; negative and NaN radii are errors."""
+    p = scenes.box_scene(3000, seed=8).xyzw()
+    p[100:110] = p[:10]                               # exact duplicates
+    q = np.ascontiguousarray(p[:40])
+    t = _tree(rm_ctx, p)
+    total, idx, d2, cnt = t.SearchRadius(_dev(q), 0.0, 8)
+    rm_ctx.sync()
+    assert total == 0 and (cnt.cpu().numpy() == 0).all()
+    assert (idx.cpu().numpy() == -1).all() and np.isinf(d2.cpu().numpy()).all()
+    with np.errstate(over="ignore"):
+        assert np.isinf(np.float32(1e30) * np.float32(1e30))
+    total, idx, d2, cnt = t.SearchRadius(_dev(q), 1e30, 32)
+    rm_ctx.sync()
+    oi, od = ob.knn_brute(p, q, 32)
+    assert (idx.cpu().numpy() == oi).all() and (d2.cpu().numpy() == od).all()
+    assert (cnt.cpu().numpy() == 32).all() and total == 32 * q.shape[0]
+    total, _, _, cnt = t.SearchRadius(_dev(q), 1e30, 0)
+    rm_ctx.sync()
+    assert (cnt.cpu().numpy() == p.shape[0]).all() and total == p.shape[0] * q.shape[0]
+    for bad in (-0.5, float("nan")):
+        with pytest.raises(RMError):
+            t.SearchRadius(_dev(q), bad, 8)
+
+
 # ------------------------------------------------------------------------------------------
 # radii
 # ------------------------------------------------------------------------------------------
