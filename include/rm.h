@@ -170,6 +170,20 @@ int rm_batch_destroy(rm_batch* batch);
 int rm_render_batch(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
                     float* rgba, int32_t* hit);
 
+/* Allocation-free form of rm_render_batch. It has no counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same outputs as rm_render_batch (default tune, one
+ * launch of the persistent march, no graph), but the tile-queue counter and the view table live
+ * in a static device array of the library, one per GPU, sized for RM_NOALLOC_MAX_VIEWS views
+ * (256 B + 160 B per view, 160.25 KiB); it calls no hipMalloc, hipFree or allocating library
+ * routine. More views than that, or a cloud with culling enabled (its culling data grow with
+ * the cloud), are rejected with RM_ERR_UNSUPPORTED. Every call on a GPU shares that GPU's
+ * array and overwrites it: calls ordered on one stream are safe, calls that overlap on one GPU
+ * (other streams or contexts) race. */
+#define RM_NOALLOC_MAX_VIEWS 1024
+int rm_render_batch_noalloc(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
+                            float* rgba, int32_t* hit);
+
 /* ---- differentiable march (no counterpart in the reference) ----------------------------- */
 
 /* Forward of the differentiable form for one view: the plain march picks the hit point h of

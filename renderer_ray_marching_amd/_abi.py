@@ -19,6 +19,7 @@ RM_ERR_NOMEM = -4
 RM_ERR_INTERNAL = -5
 RM_MAX_K = 32
 RM_MARCH_K = 10
+RM_NOALLOC_MAX_VIEWS = 1024
 
 
 class rm_view(C.Structure):
@@ -72,6 +73,7 @@ SIGNATURES = {
     "rm_batch_render_diff": (_i32, [_vp, _vp, _vp]),
     "rm_batch_destroy": (_i32, [_vp]),
     "rm_render_batch": (_i32, [_vp, _vp, C.POINTER(rm_view), _i32, _vp, _vp]),
+    "rm_render_batch_noalloc": (_i32, [_vp, _vp, C.POINTER(rm_view), _i32, _vp, _vp]),
     "rm_render_diff": (_i32, [_vp, _vp, C.POINTER(_f32), _f32, _i32, _i32, _vp, _vp, _vp]),
     "rm_render_adjoint": (_i32, [_vp, _vp, C.POINTER(_f32), _f32, _i32, _i32, _vp, _vp, _vp,
                                  _vp, _vp, _vp]),

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -17,6 +18,15 @@
 using rm::DevCloud;
 using rm::DevTree;
 using rm::DevView;
+
+// Scratch of rm_render_batch_noalloc: the tile-queue counter (first 256 bytes), then the view
+// table of at most RM_NOALLOC_MAX_VIEWS views. A file-scope device array: every GPU the library
+// runs on holds its own copy, at its own address, until the process ends. In host code the name
+// `scratch` is only a stand-in for the symbol: kernels, copies and fills use the device address
+// that hipGetSymbolAddress returns for the current device (scratch_ptr below), never the name.
+constexpr size_t kScratchHeader = 256;
+constexpr size_t kScratchBytes = kScratchHeader + (size_t)RM_NOALLOC_MAX_VIEWS * sizeof(DevView);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 struct rm_ctx {
     int device = 0;
@@ -30,6 +40,7 @@ struct rm_ctx {
     int num_cus = 0;
     void* ws = nullptr;            // grow-only workspace
     size_t ws_bytes = 0;
+    unsigned char* scratch = nullptr;  // device address of `scratch` on this ctx's device
 };
 
 struct rm_tree {
@@ -164,6 +175,24 @@ int set_device(rm_ctx* ctx) {
     return RM_OK;
 }
 
+// Device address of `scratch` on each device, looked up once per device with that device
+// current (at the first context created on it); every context on the device shares it.
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_addr;
+
+hipError_t scratch_lookup(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)device >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
+
 int ensure_ws(rm_ctx* ctx, size_t bytes) {
     if (bytes <= ctx->ws_bytes) return RM_OK;
     if (ctx->ws) {
@@ -235,6 +264,14 @@ int rm_ctx_create(int device, rm_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
     if (ctx->num_cus <= 0) ctx->num_cus = 256;
+    // the allocation-free entry point's scratch; on failure that entry point reports the error
+    // and every other function works as before
+    const hipError_t se = scratch_lookup(device, &ctx->scratch);
+    if (se != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->scratch = nullptr;
+        ctx->err = std::string("scratch lookup: ") + hipGetErrorString(se);
+    }
     *out = ctx;
     return RM_OK;
 }
@@ -945,6 +982,74 @@ int rm_render_batch(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
     r = rm_batch_launch(b);
     int r2 = rm_batch_destroy(b);
     return r != RM_OK ? r : r2;
+}
+
+int rm_render_batch_noalloc(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
+                            float* rgba, int32_t* hit) {
+    if (!ctx) return RM_ERR_INVALID;
+    if (!cloud || !views || nviews <= 0 || !rgba)
+        return fail(ctx, RM_ERR_INVALID, "rm_render_batch_noalloc: bad arguments");
+    if (nviews > RM_NOALLOC_MAX_VIEWS)
+        return fail(ctx, RM_ERR_UNSUPPORTED,
+                    "rm_render_batch_noalloc: " + std::to_string(nviews) +
+                        " views exceed RM_NOALLOC_MAX_VIEWS (" +
+                        std::to_string(RM_NOALLOC_MAX_VIEWS) + ")");
+    for (int i = 0; i < nviews; ++i) {
+        const rm_view& v = views[i];
+        if (v.width <= 0 || v.height <= 0 || v.offset < 0 ||
+            !(v.fov_radians > 0.0f && v.fov_radians < 3.14159f))
+            return fail(ctx, RM_ERR_INVALID,
+                        "rm_render_batch_noalloc: bad view " + std::to_string(i));
+    }
+    if (int r = check_cloud(ctx, cloud)) return r;
+    if (cloud->cull)
+        return fail(ctx, RM_ERR_UNSUPPORTED,
+                    "rm_render_batch_noalloc: culling data grow with the cloud and views; "
+                    "use rm_render_batch for a culled cloud");
+    if (!ctx->scratch)
+        return fail(ctx, RM_ERR_HIP,
+                    "rm_render_batch_noalloc: the scratch address lookup failed at context "
+                    "creation");
+    if (int r = set_device(ctx)) return r;
+    rm_tune t = default_tune();
+    if (cloud->n < rm::kMarchK) t.warm = 0;
+    // The view table goes to the scratch array through kernel arguments, kViewChunk views per
+    // launch: no staging buffer, and ordered on the stream like the march itself.
+    DevView* d_views = reinterpret_cast<DevView*>(ctx->scratch + kScratchHeader);
+    int64_t tiles = 0;
+    rm::ViewChunk ch;
+    for (int i0 = 0; i0 < nviews; i0 += rm::kViewChunk) {
+        ch.first = i0;
+        ch.count = std::min(rm::kViewChunk, nviews - i0);
+        for (int j = 0; j < ch.count; ++j) {
+            const rm_view& v = views[i0 + j];
+            ch.v[j] = dev_view(v.pose, v.fov_radians, v.width, v.height, v.offset, t.tile_w, tiles);
+            tiles += (int64_t)ch.v[j].tiles_x * ((v.height + t.tile_h - 1) / t.tile_h);
+        }
+        if (tiles >= (int64_t)UINT32_MAX)
+            return fail(ctx, RM_ERR_UNSUPPORTED, "rm_render_batch_noalloc: too many tiles");
+        RM_HIP(ctx, rm::launch_store_views(ctx->stream, ch, d_views));
+    }
+    rm::BatchParams p;
+    p.cloud = dev_cloud(cloud);
+    p.cull_mask = nullptr;
+    p.cull_words = 0;
+    p.views = d_views;
+    p.nviews = nviews;
+    p.tile_w = t.tile_w;
+    p.tile_h = t.tile_h;
+    p.refill_min = t.refill;
+    p.total_tiles = tiles;
+    p.queue = reinterpret_cast<unsigned long long*>(ctx->scratch);
+    p.err = ctx->d_err;
+    p.out.rgba = reinterpret_cast<float4*>(rgba);
+    p.out.hit = hit;
+    timing_begin(ctx);
+    RM_HIP(ctx, rm::launch_reset_queue(ctx->stream, p.queue));
+    RM_HIP(ctx, rm::launch_batch(ctx->stream, p, t.mode, t.warm, t.waves_per_block,
+                                 ctx->num_cus * std::max(1, 32 / t.waves_per_block)));
+    timing_end(ctx);
+    return RM_OK;
 }
 
 // ---------------------------------------------------------------------------------------------

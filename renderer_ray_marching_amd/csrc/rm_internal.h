@@ -60,6 +60,16 @@ struct DevView {
 static_assert(offsetof(DevView, fdir) == offsetof(DevView, fn) + 12 * sizeof(float),
               "the kernels read fn and fdir as one run of 15 floats");
 
+// Views handed to a kernel by value (kernel arguments stay within 4 KiB): the allocation-free
+// batched march fills its view table with one launch per chunk.
+constexpr int kViewChunk = 24;
+struct ViewChunk {
+    DevView v[kViewChunk];
+    int32_t first;          // index of v[0] in the table
+    int32_t count;          // views used, 1..kViewChunk
+};
+static_assert(sizeof(ViewChunk) <= 4096, "kernel arguments are limited to 4 KiB");
+
 struct MarchOut {
     float4* rgba;
     int32_t* hit;           // may be null
@@ -112,6 +122,7 @@ hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& 
 hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
                         int blocks);
 hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q);
+hipError_t launch_store_views(hipStream_t s, const ViewChunk& ch, DevView* table);
 hipError_t launch_cull_masks(hipStream_t s, const DevTree& t, const DevView* views, int nviews,
                              uint32_t* cull, int64_t stride, const TreeLevels& lv);
 hipError_t launch_refit(hipStream_t s, RmNode* nodes, float4* pts, float4* xyzr,

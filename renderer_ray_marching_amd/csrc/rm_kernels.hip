@@ -922,6 +922,11 @@ __global__ __launch_bounds__(256) void cull_nodes_kernel(const RmNode* __restric
 
 __global__ void reset_queue_kernel(unsigned long long* q) { *q = 0ull; }
 
+__global__ void store_views_kernel(ViewChunk ch, DevView* __restrict__ table) {
+    const int i = threadIdx.x;
+    if (i < ch.count) table[ch.first + i] = ch.v[i];
+}
+
 // ------------------------------------------------------------------------------------------
 // Cloud set-up and refit
 // ------------------------------------------------------------------------------------------
@@ -1429,6 +1434,11 @@ hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm,
 
 hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q) {
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_store_views(hipStream_t s, const ViewChunk& ch, DevView* table) {
+    hipLaunchKernelGGL(store_views_kernel, dim3(1), dim3(kViewChunk), 0, s, ch, table);
     return hipGetLastError();
 }
 
