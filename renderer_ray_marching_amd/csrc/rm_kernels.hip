@@ -626,16 +626,35 @@ template <bool CULL = false>
 __device__ __forceinline__ void warm_seed(const DevCloud& c, TopK<kMarchK>& tk, float qx,
                                           float qy, float qz) {
     const float* xyzr = (const float*)c.xyzr;
+    uint32_t ids[kMarchK];
 #pragma unroll
-    for (int j = 0; j < kMarchK; ++j) {
-        const uint32_t id = key_idx(tk.key[j]);
-        if (CULL && id >= (uint32_t)c.tree.n) {
-            tk.key[j] = kPadKey;
-            continue;
+    for (int j = 0; j < kMarchK; ++j) ids[j] = key_idx(tk.key[j]);
+    // Two batches of five gathers. With all ten in flight at once, the 30 coordinates and the
+    // list do not fit the register budget of 8 wavefronts per SIMD and the list is spilled;
+    // the empty asm makes the second batch's indices depend on the first batch's keys, so the
+    // second batch is issued only once the first is consumed.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (h == 1)
+            asm volatile("" : "+v"(ids[5]), "+v"(ids[6]), "+v"(ids[7]), "+v"(ids[8]), "+v"(ids[9])
+                         : "v"(tk.key[0]), "v"(tk.key[1]), "v"(tk.key[2]), "v"(tk.key[3]),
+                           "v"(tk.key[4]));
+#pragma unroll
+        for (int j = 5 * h; j < 5 * h + 5; ++j) {
+            const uint32_t id = ids[j];
+            if (CULL && id >= (uint32_t)c.tree.n) {
+                tk.key[j] = kPadKey;
+                continue;
+            }
+            const float* q = xyzr + 4 * (size_t)id;   // x, y, z only: the radius is not needed
+            tk.key[j] = make_key(dist2(q[0], q[1], q[2], qx, qy, qz), id);
         }
-        const float* q = xyzr + 4 * (size_t)id;   // x, y, z only: the radius is not needed
-        tk.key[j] = make_key(dist2(q[0], q[1], q[2], qx, qy, qz), id);
     }
+    // the sort starts once every key is formed (its first exchanges would otherwise overlap
+    // the second batch of gathers and raise the register peak again)
+    asm volatile("" : "+v"(tk.key[0]), "+v"(tk.key[1]), "+v"(tk.key[2]), "+v"(tk.key[3]),
+                 "+v"(tk.key[4]), "+v"(tk.key[5]), "+v"(tk.key[6]), "+v"(tk.key[7]),
+                 "+v"(tk.key[8]), "+v"(tk.key[9]));
     static_assert(kMarchK == 10, "sort10 sorts ten keys");
     sort10(tk.key);
 }
@@ -713,35 +732,66 @@ __global__ __launch_bounds__(256) void render_plain_kernel(DevCloud c, DevView v
 //
 // Occupancy: each traversal step waits on scalar loads of node records and points and on the
 // LDS stack, so the vector ALUs need many resident wavefronts to stay busy. The kernel is
-// compiled for 8 wavefronts per SIMD (64 VGPRs); the values that do not fit are per-ray state
-// spilled to scratch outside the traversal loops (measured 17 % faster than 5 wavefronts per SIMD
-// at 91 VGPRs without spills; the vector ALUs then issue 90 % of the time).
+// compiled for 8 wavefronts per SIMD (64 VGPRs). The per-ray state that is not needed inside the
+// traversal (direction, distance marched, far limit, step count, pixel) lives in LDS between the
+// points where it is used, so the traversal keeps the registers and nothing goes to scratch.
+template <int WPB>
+struct RayStateLds {
+    float dx[WPB][64], dy[WPB][64], dz[WPB][64], t[WPB][64], tfar[WPB][64];
+    int steps[WPB][64], pix[WPB][64];
+};
+
+// Compiler barrier: values read from LDS after it are read again (not kept in registers from
+// an earlier read or store), which keeps the per-ray state out of the registers across the
+// traversal.
+__device__ __forceinline__ void reread_fence() { asm volatile("" ::: "memory"); }
+
+// v unchanged, but opaque to the compiler: what is computed from it stays where it is written
+// instead of being hoisted out of the loop (and held in a register across the traversal).
+__device__ __forceinline__ int opaque_s(int v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+// kPadKey, formed where it is used rather than kept in a register pair across the march
+__device__ __forceinline__ uint64_t pad_key_here() {
+    uint32_t lo = 0xffffffffu, hi = 0x7f800000u;
+    asm volatile("" : "+v"(lo), "+v"(hi));
+    return ((uint64_t)hi << 32) | lo;
+}
+template <class T>
+__device__ __forceinline__ T* opaque_ptr(T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 template <int MODE, bool WARM, int WPB, bool CULL>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) void batch_kernel(
     BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
     __shared__ int lq[WPB][kLeafQueue];
+    __shared__ RayStateLds<WPB> rs;
     extern __shared__ int lstk[];
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     CFloat* vf = (CFloat*)p.views;
     constexpr int VSTRIDE = sizeof(DevView) / 4;
-    const int tile_w = p.tile_w, tile_pix = p.tile_w * p.tile_h;
+    const int tile_pix = p.tile_w * p.tile_h;
     while (true) {
         unsigned long long qv = 0;
         if (lane == 0) qv = atomicAdd(p.queue, 1ull);
         qv = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(qv >> 32)) << 32) |
              (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qv);
-        const int64_t tile = (int64_t)qv;
-        if (tile >= p.total_tiles) break;
+        // the host keeps total_tiles below 2^32, so tiles are compared in 32 bits (the scalar
+        // unit has no ordered 64-bit compare)
+        if ((uint32_t)(qv >> 32) != 0u || (uint32_t)qv >= (uint32_t)p.total_tiles) break;
+        const int64_t tile = (int64_t)(uint32_t)qv;
         // view lookup: last view whose tile_offset <= tile (scalar loads)
         int lo = 0, hi = p.nviews - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;
-            int64_t off = *(const __attribute__((address_space(4))) int64_t*)(
+            const uint32_t off = *(const __attribute__((address_space(4))) uint32_t*)(
                 vf + (int64_t)mid * VSTRIDE + offsetof(DevView, tile_offset) / 4);
-            if (off <= tile)
+            if (off <= (uint32_t)tile)
                 lo = mid;
             else
                 hi = mid - 1;
@@ -755,34 +805,41 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
         const int64_t tile_off =
             *(const __attribute__((address_space(4))) int64_t*)(V + offsetof(DevView, tile_offset) / 4);
         const int64_t lt = tile - tile_off;
-        const int x0 = (int)(lt % tiles_x) * tile_w;
+        const int x0 = (int)(lt % tiles_x) * p.tile_w;
         const int y0 = (int)(lt / tiles_x) * p.tile_h;
         // camera origin: the same for every lane of the tile (scalar registers)
         const float ox = V[12], oy = V[13], oz = V[14];
         const uint32_t* vmask = CULL ? p.cull_mask + (int64_t)lo * p.cull_words : nullptr;
         int next = 0;                      // wave-uniform: next unclaimed pixel of the tile
         bool has = false;
-        // per-lane ray state: direction, far limit, distance marched, steps, pixel in the view
-        float dx = 0.f, dy = 0.f, dz = -1.f, tfar = 0.f, t = 0.f;
-        int steps = 0, pix = 0;
         TopK<kMarchK> tk;
         tk.init(kPadKey);
         while (true) {
             const uint64_t needm = __builtin_amdgcn_ballot_w64(!has);
             const int nneed = __popcll(needm);
             if (next < tile_pix && (nneed >= p.refill_min || nneed == 64)) {
-                const int my = next + __popcll(needm & lt_mask);
+                // idle lanes below this one (mbcnt counts the mask's bits under the lane)
+                const int my = next + (int)__builtin_amdgcn_mbcnt_hi(
+                                          (uint32_t)(needm >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)needm, 0u));
                 if (!has && my < tile_pix) {
-                    const int x = x0 + my % tile_w, y = y0 + my / tile_w;
-                    if (x < W && y < H) {
-                        const Ray r = make_ray([&](int i) { return V[i]; }, pane, W, H, x, y);
-                        dx = r.dx;
-                        dy = r.dy;
-                        dz = r.dz;
-                        tfar = fminf(r.tfar, ray_exit_bound(p.cloud, r));
-                        t = r.tnear;
-                        steps = 0;
-                        pix = y * W + x;
+                    // the tile and view values are made opaque here so that what the refill
+                    // derives from them is computed here, not kept live across the march
+                    const int tw = opaque_s(p.tile_w), Wr = opaque_s(W), Hr = opaque_s(H);
+                    const int x = x0 + my % tw, y = y0 + my / tw;
+                    if (x < Wr && y < Hr) {
+                        CFloat* Vr = opaque_ptr(V);
+                        DevCloud cl = p.cloud;
+                        cl.tree.nodes = opaque_ptr(cl.tree.nodes);
+                        cl.rmax = opaque_ptr(cl.rmax);
+                        const Ray r = make_ray([&](int i) { return Vr[i]; }, Vr[16], Wr, Hr, x, y);
+                        rs.dx[w][lane] = r.dx;
+                        rs.dy[w][lane] = r.dy;
+                        rs.dz[w][lane] = r.dz;
+                        rs.t[w][lane] = r.tnear;
+                        rs.tfar[w][lane] = fminf(r.tfar, ray_exit_bound(cl, r));
+                        rs.steps[w][lane] = 0;
+                        rs.pix[w][lane] = y * Wr + x;
                         has = true;
                     }
                 }
@@ -792,16 +849,22 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                 if (next >= tile_pix) break;
                 continue;
             }
-            const float px = ox + dx * t;
-            const float py = oy + dy * t;
-            const float pz = oz + dz * t;
-            const bool warm_lane = WARM && has && steps > 0;
+            reread_fence();
+            float px = 0.f, py = 0.f, pz = 0.f;
+            bool warm_lane = false;
+            if (has) {
+                const float t = rs.t[w][lane];
+                px = ox + rs.dx[w][lane] * t;
+                py = oy + rs.dy[w][lane] * t;
+                pz = oz + rs.dz[w][lane] * t;
+                warm_lane = WARM && rs.steps[w][lane] > 0;
+            }
             if (!has)
                 tk.init(0ull);
             else if (warm_lane)
                 warm_seed<CULL>(p.cloud, tk, px, py, pz);
             else
-                tk.init(kPadKey);
+                tk.init(pad_key_here());
             if (MODE == 0) {
                 // a fresh lane's pads (index 0xffffffff) never match a real candidate, so the
                 // duplicate check is harmless for it and one traversal serves both kinds
@@ -815,17 +878,21 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                     knn_lane<false, CULL>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
                                           WPB * 64, maxsp, p.err, vmask);
             }
+            reread_fence();
             if (has) {
                 float s;
                 uint32_t id;
                 sdf10(p.cloud, tk, s, id);
-                t = t + s;
-                ++steps;
+                const float t = rs.t[w][lane] + s;
+                const int steps = rs.steps[w][lane] + 1;
                 const bool hitnow = s < kMinDist;
-                if (hitnow || t > tfar || steps >= kMaxSteps) {
-                    write_pixel(p.cloud, p.out, pix_off + pix, hitnow ? (int)id : -1,
+                if (hitnow || t > rs.tfar[w][lane] || steps >= kMaxSteps) {
+                    write_pixel(p.cloud, p.out, pix_off + rs.pix[w][lane], hitnow ? (int)id : -1,
                                 hitnow ? t : 0.0f);
                     has = false;
+                } else {
+                    rs.t[w][lane] = t;
+                    rs.steps[w][lane] = steps;
                 }
             }
         }
