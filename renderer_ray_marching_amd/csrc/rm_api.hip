@@ -23,7 +23,8 @@ using rm::DevView;
 // table of at most RM_NOALLOC_MAX_VIEWS views. A file-scope device array: every GPU the library
 // runs on holds its own copy, at its own address, until the process ends. In host code the name
 // `scratch` is only a stand-in for the symbol: kernels, copies and fills use the device address
-// that hipGetSymbolAddress returns for the current device (scratch_ptr below), never the name.
+// that hipGetSymbolAddress returns for the current device (kept per device by scratch_lookup
+// below), never the name.
 constexpr size_t kScratchHeader = 256;
 constexpr size_t kScratchBytes = kScratchHeader + (size_t)RM_NOALLOC_MAX_VIEWS * sizeof(DevView);
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -51,6 +52,7 @@ struct rm_tree {
     int leaf_max = 0;
     RmNode* nodes = nullptr;
     float4* pts = nullptr;
+    int32_t* inv = nullptr;                // leaf position of each original index (device)
     int32_t* level_order = nullptr;        // inner node ids grouped by level (device)
     std::vector<int32_t> level_start;      // host: offsets into level_order, nlevels + 1
 };
@@ -106,6 +108,7 @@ DevTree dev_tree(const rm_tree* t) {
     DevTree d;
     d.nodes = t->nodes;
     d.pts = t->pts;
+    d.inv = t->inv;
     d.n = t->n;
     d.depth = t->depth;
     d.leaf_max = t->leaf_max;
@@ -356,6 +359,7 @@ int tree_build_host(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_
     }
     if (hipMalloc(&t->nodes, sizeof(RmNode) * ht.nodes.size()) != hipSuccess ||
         hipMalloc(&t->pts, sizeof(float4) * (size_t)n) != hipSuccess ||
+        hipMalloc(&t->inv, sizeof(int32_t) * (size_t)n) != hipSuccess ||
         hipMalloc(&t->level_order, sizeof(int32_t) * std::max<size_t>(1, order.size())) !=
             hipSuccess) {
         (void)hipGetLastError();
@@ -368,6 +372,7 @@ int tree_build_host(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_
     if (!order.empty())
         RM_HIP(ctx, hipMemcpyAsync(t->level_order, order.data(), sizeof(int32_t) * order.size(),
                                    hipMemcpyHostToDevice, ctx->stream));
+    RM_HIP(ctx, rm::launch_tree_inverse(ctx->stream, t->pts, n, t->inv));
     RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RM_OK;
 }
@@ -388,6 +393,7 @@ int tree_build_device(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, r
     for (size_t i = 0; i < nn; ++i) order[i] = (int32_t)i;  // breadth-first = level order
     if (hipMalloc(&t->nodes, sizeof(RmNode) * nn) != hipSuccess ||
         hipMalloc(&t->pts, sizeof(float4) * (size_t)n) != hipSuccess ||
+        hipMalloc(&t->inv, sizeof(int32_t) * (size_t)n) != hipSuccess ||
         hipMalloc(&t->level_order, sizeof(int32_t) * nn) != hipSuccess) {
         (void)hipGetLastError();
         return fail(ctx, RM_ERR_NOMEM, "rm_tree_build: device allocation failed");
@@ -443,6 +449,7 @@ int tree_build_device(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, r
                            ctx->stream);
     if (e == hipSuccess)
         e = rm::build_tree_device(ctx->stream, in, shape, seg_b, seg_e, nl, t->nodes, t->pts, w);
+    if (e == hipSuccess) e = rm::launch_tree_inverse(ctx->stream, t->pts, n, t->inv);
     hipError_t e2 = hipStreamSynchronize(ctx->stream);
     (void)hipFree(base);
     if (e != hipSuccess) return hip_fail(ctx, e, "rm_tree_build (device)");
@@ -482,6 +489,7 @@ int rm_tree_destroy(rm_tree* t) {
     (void)hipSetDevice(t->device);
     if (t->nodes) (void)hipFree(t->nodes);
     if (t->pts) (void)hipFree(t->pts);
+    if (t->inv) (void)hipFree(t->inv);
     if (t->level_order) (void)hipFree(t->level_order);
     delete t;
     return RM_OK;
@@ -536,8 +544,10 @@ int rm_radii(rm_ctx* ctx, const rm_tree* tree, float max_radius, float* r_unc, f
     if (int r = check_tree(ctx, tree)) return r;
     if (int r = set_device(ctx)) return r;
     if (!r_unc && !r_cl) return RM_OK;
+    if (int r = ensure_ws(ctx, sizeof(float) * (size_t)tree->n)) return r;
     timing_begin(ctx);
-    RM_HIP(ctx, rm::launch_radii(ctx->stream, dev_tree(tree), max_radius, r_unc, r_cl, ctx->d_err));
+    RM_HIP(ctx, rm::launch_radii(ctx->stream, dev_tree(tree), max_radius, r_unc, r_cl,
+                                 static_cast<float*>(ctx->ws), ctx->d_err));
     timing_end(ctx);
     return RM_OK;
 }

@@ -31,6 +31,7 @@ constexpr int kLeafQueue = RM_LEAF_QUEUE;
 struct DevTree {
     const RmNode* nodes;
     const float4* pts;      // leaf order: x, y, z, original index bits
+    const int32_t* inv;     // original order: leaf position of each point
     int64_t n;
     int depth;
     int leaf_max;           // every leaf holds leaf_max points except the last one
@@ -110,8 +111,11 @@ hipError_t launch_knn(hipStream_t s, const DevTree& t, const float* q, int64_t m
                       int32_t* idx, float* d2, int* err);
 hipError_t launch_radius(hipStream_t s, const DevTree& t, const float* q, int64_t m, float r2,
                          int max_nn, int32_t* idx, float* d2, int32_t* counts, int* err);
+// tmp: n floats of scratch (the radii in leaf order before they are put in original order)
 hipError_t launch_radii(hipStream_t s, const DevTree& t, float max_radius, float* r_unc,
-                        float* r_cl, int* err);
+                        float* r_cl, float* tmp, int* err);
+// inv[original index of pts[i]] = i
+hipError_t launch_tree_inverse(hipStream_t s, const float4* pts, int64_t n, int32_t* inv);
 hipError_t launch_make_cloud(hipStream_t s, const float* xyzw, const float* radii,
                              const float* colors, int64_t n, float4* xyzr, float4* col);
 hipError_t launch_set_radii(hipStream_t s, const float* radii, int64_t n, float4* xyzr);

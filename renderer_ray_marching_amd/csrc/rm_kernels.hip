@@ -476,11 +476,13 @@ __global__ __launch_bounds__(256) void radius_count_kernel(DevTree t, const floa
 }
 
 // Radii: one wavefront per 64 consecutive leaf-ordered points (spatially coherent), packet
-// traversal with k = 2, then sqrt + clamp fused, scattered to the original order.
+// traversal with k = 2, then the square root, written in leaf order (coalesced). The original
+// order is one random permutation of the leaf order, so a scatter would pay a 32-byte sector
+// per 4-byte value; radii_order_kernel instead gathers each original index's value and writes
+// both outputs coalesced.
 template <int WPB>
-__global__ __launch_bounds__(WPB * 64) void radii_kernel(DevTree t, float max_radius,
-                                                         float* __restrict__ r_unc,
-                                                         float* __restrict__ r_cl, int* err) {
+__global__ __launch_bounds__(WPB * 64) void radii_kernel(DevTree t, float* __restrict__ r_leaf,
+                                                         int* err) {
     __shared__ int wstk[WPB][kWaveStack];
     int w = threadIdx.x >> 6;
     const int64_t g0 = ((int64_t)xcd_block() * WPB + w) * 64;  // first point of the group
@@ -496,12 +498,27 @@ __global__ __launch_bounds__(WPB * 64) void radii_kernel(DevTree t, float max_ra
     const int gcnt = (64 % t.leaf_max == 0) ? (int)min((int64_t)64, t.n - g0) : 0;
     packet_scan_leaf<2, false>((CFloat*)t.pts, (int)g0, gcnt, q.x, q.y, q.z, tk);
     knn_packet<2>(t, q.x, q.y, q.z, active, tk, wstk[w], err, (int)g0, (int)(g0 + gcnt));
-    if (active) {
-        float r = sqrtf(key_d2(tk.key[1]));
-        uint32_t o = __float_as_uint(q.w);
-        if (r_unc) r_unc[o] = r;
-        if (r_cl) r_cl[o] = (max_radius > 0.0f && r > max_radius) ? max_radius : r;
-    }
+    if (active) r_leaf[pos] = sqrtf(key_d2(tk.key[1]));
+}
+
+// Original order: r_unc[o] = r_leaf[inv[o]], r_cl[o] = the same clamped to max_radius (when
+// max_radius > 0). Reads of inv and writes are coalesced; the gathers hit a table of 4 n bytes.
+__global__ __launch_bounds__(256) void radii_order_kernel(const int32_t* __restrict__ inv,
+                                                          const float* __restrict__ r_leaf,
+                                                          int64_t n, float max_radius,
+                                                          float* __restrict__ r_unc,
+                                                          float* __restrict__ r_cl) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const float r = r_leaf[inv[o]];
+    if (r_unc) r_unc[o] = r;
+    if (r_cl) r_cl[o] = (max_radius > 0.0f && r > max_radius) ? max_radius : r;
+}
+
+__global__ void tree_inverse_kernel(const float4* __restrict__ pts, int64_t n,
+                                    int32_t* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[__float_as_uint(pts[i].w)] = (int32_t)i;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1391,13 +1408,22 @@ hipError_t launch_radius(hipStream_t s, const DevTree& t, const float* q, int64_
 }
 
 hipError_t launch_radii(hipStream_t s, const DevTree& t, float max_radius, float* r_unc,
-                        float* r_cl, int* err) {
+                        float* r_cl, float* tmp, int* err) {
     // one wavefront per block: measured 5 % faster than 2 or 4 (finer scheduling, XCD-contiguous
     // runs of single groups), same results
     constexpr int WPB = 1;
     int64_t waves = (t.n + 63) / 64;
     hipLaunchKernelGGL(radii_kernel<WPB>, dim3(grid_for(waves, WPB)), dim3(WPB * 64), 0, s, t,
-                       max_radius, r_unc, r_cl, err);
+                       tmp, err);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(radii_order_kernel, dim3(grid_for(t.n, 256)), dim3(256), 0, s, t.inv, tmp,
+                       t.n, max_radius, r_unc, r_cl);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_inverse(hipStream_t s, const float4* pts, int64_t n, int32_t* inv) {
+    hipLaunchKernelGGL(tree_inverse_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, pts, n, inv);
     return hipGetLastError();
 }
 

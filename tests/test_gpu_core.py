@@ -277,6 +277,32 @@ def test_radii_duplicates_lattice_single(rm_ctx):
     assert np.isinf(ru1.cpu().numpy()[0]) and rc1.cpu().numpy()[0] == np.float32(0.1)
 
 
+def test_radii_single_output_and_after_refit(rm_ctx):
+    """rm_radii with one output only (the other NULL), and on a tree whose points were moved by
+    a refit (same leaf order, so the leaf-to-original mapping still holds)."""
+    c = scenes.box_scene(30_000, seed=11)
+    p = c.xyzw()
+    n = p.shape[0]
+    t = _tree(rm_ctx, p)
+    L = _abi.lib()
+    ru = torch.full((n,), -1.0, device="cuda")
+    rc = torch.full((n,), -1.0, device="cuda")
+    _abi.check(L.rm_radii(rm_ctx.h, t.h, 0.02, api._ptr(ru), None), rm_ctx.h)
+    _abi.check(L.rm_radii(rm_ctx.h, t.h, 0.02, None, api._ptr(rc)), rm_ctx.h)
+    rm_ctx.sync()
+    oru, orc, _ = ob.RefTree(p, 64).radii(0.02)
+    assert (ru.cpu().numpy() == oru).all() and (rc.cpu().numpy() == orc).all()
+    # move every point, refit through a cloud, recompute the radii of the moved cloud
+    moved = p.copy()
+    moved[:, :3] += np.random.default_rng(5).normal(0, 0.01, (n, 3)).astype(np.float32)
+    m = api.PointcloudRayMarcher.get_instance(_dev(p), _dev(c.colors4()), rc, t)
+    m.refit(_dev(np.ascontiguousarray(moved[:, :3])))
+    ru2, rc2 = t.radii(0.02)
+    rm_ctx.sync()
+    oru2, orc2, _ = ob.RefTree(moved, 64).radii(0.02)
+    assert (ru2.cpu().numpy() == oru2).all() and (rc2.cpu().numpy() == orc2).all()
+
+
 # ------------------------------------------------------------------------------------------
 # plain march
 # ------------------------------------------------------------------------------------------
@@ -681,11 +707,63 @@ def test_full_size_room_10m_256_views(rm_ctx):
     assert n > 100_000
 
 
+@pytest.fixture(scope="module")
+def scale50m():
+    """50M points in a 40 x 30 x 4 m room (1 % outliers), 1024 views at 1920x1080."""
+    return scenes.workload_scale_50m()
+
+
 @pytest.mark.slow
-def test_full_size_scale_50m_view_subset(rm_ctx):
+@pytest.mark.timeout(600)
+def test_full_size_scale_50m_view_subset(rm_ctx, scale50m):
     """The largest configuration's cloud (50M points, radii against the oracle on all of them)
     with every 64th of its 1024 views at 1920x1080 (the batch output of all 1024 views would be
     42 GB on the host)."""
-    cloud, views, max_r = scenes.workload_scale_50m()
+    cloud, views, max_r = scale50m
     n = _full_size_check(rm_ctx, cloud, views[::64], max_r, [0, 5, 10, 15], 64)
     assert n > 100_000
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_full_size_scale_50m_culled_1024_views_one_launch(rm_ctx, scale50m):
+    """The largest configuration as stated: all 1024 views at 1920x1080 (2.12G rays) in ONE
+    persistent launch, each view marching over its frustum-culled share of the 50M points. The
+    42 GB output stays on the device: four views (0, 341, 682, 1023) are compared with the
+    oracle's culled march (a tree over each view's inside points) on the (x + 7 y) mod 64 pixels,
+    and two views with the culled plain kernel on every pixel."""
+    cloud, views, max_r = scale50m
+    print("case: scale50m, 1024 culled views in one batched launch", flush=True)
+    p = cloud.xyzw()
+    t = _tree(rm_ctx, p)
+    _, rc = t.radii(max_r)
+    m = api.PointcloudRayMarcher(_dev(p), _dev(cloud.colors4()), rc, t)
+    m.set_culling(True)
+    b = api.Batch(m, views)
+    assert b.total == sum(v.width * v.height for v in views) == 2_123_366_400
+    b.launch()
+    rm_ctx.sync()
+    offs = np.cumsum([0] + [v.width * v.height for v in views])
+    rch, col = rc.cpu().numpy(), cloud.colors4()
+    checked = 0
+    for i in (0, 341, 682, 1023):
+        v = views[i]
+        pix = _subsample(v.width, v.height, 64)
+        sel = torch.from_numpy(offs[i] + pix.astype(np.int64)).cuda()
+        ghit, grgba = b.hit[sel].cpu().numpy(), b.rgba[sel].cpu().numpy()
+        orgba, ohit, inside = ob.render_culled(p, rch, col, v.pose_colmajor(), v.fov, v.width,
+                                               v.height, pix=pix)
+        assert 0 < inside.sum() < p.shape[0], i
+        assert (ghit == ohit).all(), i
+        assert (grgba == orgba).all(), i
+        checked += pix.size
+    for i in (0, 1023):
+        v = views[i]
+        n = v.width * v.height
+        prgba, phit = m.render_to_texture(v.pose_colmajor(), v.fov, v.width, v.height)
+        rm_ctx.sync()
+        assert torch.equal(phit.reshape(-1), b.hit[offs[i]:offs[i] + n]), i
+        assert torch.equal(prgba.reshape(-1, 4), b.rgba[offs[i]:offs[i] + n]), i
+    assert checked > 100_000
+    assert float((b.hit >= 0).float().mean().item()) > 0.5
+    b.close()
