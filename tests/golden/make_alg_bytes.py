@@ -1,6 +1,11 @@
-"""Counts the algorithmic bytes of the measured workloads with the oracle's restatement of the
-reference traversal (kdtree_cuda_3d_index.cu:52-129) on the reference tree rule (leaf 64), and
-writes them to tests/golden/alg_bytes.json for bench.py's roofline.
+"""Counts the algorithmic bytes of the measured workloads and writes them to
+tests/golden/alg_bytes.json for bench.py's work figures, in two forms:
+
+1. the reference's traversal (kdtree_cuda_3d_index.cu:52-129) on the reference tree rule
+   (leaf 64), restated by the oracle;
+2. what one lane of this library's kernels reads when it searches alone (work_model.cpp: the
+   exact per-lane kNN over the library's leaf-16 tree, the warm-started march with the early
+   miss), whose hit indices and depths must equal the oracle's on the counted pixels.
 
 Per kNN-radius query:   B_q = 16 + 12 C_q + 16 V_q + 24 A_q
 Per marched ray:        B_ray = sum over steps (12 C + 16 V + 24 A + 10 (4 + 4)) + 16 + 4
@@ -24,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 from oracle import binding as ob  # noqa: E402
 from renderer_ray_marching_amd import scenes  # noqa: E402
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import work_model as wm  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "alg_bytes.json")
 
@@ -62,6 +70,25 @@ def count(name):
         rays_total += v.width * v.height
         hits += int((hit >= 0).sum())
     march_bytes = tot.traversal_bytes() + 80 * tot.s + 20 * rays_counted
+    # this library's kernels, one lane at a time
+    wt = wm.WorkTree(p, 16)
+    wr, wrc = wt.radii()
+    assert np.array_equal(wr, ru), "work model radii differ from the oracle's"
+    wt.set_radii(rc)
+    wtot = wm.Counts()
+    for v in views:
+        pix = subsample(v.width, v.height)
+        orgba, ohit, _ = tree.render(rc, col, v.pose_colmajor(), v.fov, v.width, v.height,
+                                     pix=pix)
+        whit, wdep, c = wt.render(v.pose_colmajor(), ob.pane_distance(v.width, v.fov), v.width,
+                                  v.height, pix=pix)
+        assert (whit == ohit).all() and np.array_equal(wdep, orgba[:, 3]), v.name
+        for f, _ in wm.Counts._fields_:
+            setattr(wtot, f, getattr(wtot, f) + getattr(c, f))
+    # per ray: traversal reads, RGBA32F + hit index written, the hit point's colour read
+    this_march = wtot.traversal_bytes() + 20 * wtot.rays + 12 * wtot.hits
+    # per query: the traversal reads, the query point (16 B) read, two 4-byte radii written
+    this_radii = wrc.traversal_bytes() + 24 * cloud.n
     rec = {
         "points": cloud.n,
         "views": len(views),
@@ -75,6 +102,13 @@ def count(name):
         "march_steps_per_ray": tot.s / rays_counted,
         "march_bytes_per_ray": march_bytes / rays_counted,
         "march_hit_fraction": hits / rays_counted,
+        "this_march_counters": wtot.as_dict(),
+        "this_march_bytes_per_ray": this_march / wtot.rays,
+        "this_radii_counters": wrc.as_dict(),
+        "this_radii_bytes_per_query": this_radii / cloud.n,
+        "this_basis": "per-lane exact search of the leaf-16 tree with warm start and early miss "
+                      "(work_model.cpp); nodes and re-tested leaves 64 B, leaf points 16 B, "
+                      "warm-start gathers 12 B, radius gathers 4 B",
         "subsample": "pixels with (x + 7 y) mod 64 == 0",
         "seconds": time.time() - t0,
     }
