@@ -10,6 +10,8 @@
  *
  * Threading: one rm_ctx per host thread; calls on one ctx are serialised by its stream. There
  * is no global device state: any number of trees, clouds and batches may coexist.
+ * Lifetimes: a batch uses its context and cloud, a cloud its tree; destroy a batch before its
+ * context and cloud, and a cloud before its tree.
  */
 #ifndef RM_H
 #define RM_H
@@ -118,7 +120,9 @@ int rm_cloud_set_colors(rm_ctx* ctx, rm_cloud* cloud, const float* colors);
 int rm_cloud_set_culling(rm_ctx* ctx, rm_cloud* cloud, int enable);
 /* Move the points of a cloud (xyz: n x 3 floats, original order) and refit the boxes of the
  * tree it was created from, keeping the topology. Search stays exact because every bound is
- * recomputed from the moved points. The tree is modified in place. */
+ * recomputed from the moved points. The tree is modified in place: other clouds created on the
+ * same tree no longer match its points, and their march calls (rm_render, rm_render_diff,
+ * rm_render_adjoint, the batch calls) fail with RM_ERR_INVALID until they are refit too. */
 int rm_cloud_refit(rm_ctx* ctx, rm_cloud* cloud, const float* xyz);
 
 /* Render one view. pose: 16 floats, column-major camera-to-world (OpenGL, -z forward), as the
@@ -200,13 +204,26 @@ int rm_render_diff(rm_ctx* ctx, const rm_cloud* cloud, const float* pose, float 
  * by view. */
 int rm_batch_render_diff(rm_batch* batch, float* rgb, float* depth);
 
-/* Adjoint of rm_render_diff with the hit map held fixed. grad_rgb: H x W x 3, grad_depth:
+/* Adjoint of rm_render_diff with the hit map held fixed, at the cloud's current centres and
+ * radii. grad_rgb: H x W x 3, grad_depth:
  * H x W upstream gradients. Accumulates (adds) into g_xyz (n x 3), g_r (n), g_rgb (n x 3) in
  * the original point order. Per-point sums are formed by sorting the hit pixels by point
  * index and reducing each run, not by atomics on the point arrays. */
 int rm_render_adjoint(rm_ctx* ctx, const rm_cloud* cloud, const float* pose, float fov_radians,
                       int W, int H, const int32_t* hit, const float* grad_rgb,
                       const float* grad_depth, float* g_xyz, float* g_r, float* g_rgb);
+
+/* Adjoint of rm_batch_render_diff over every view of the batch at once (no counterpart in the
+ * reference), with the hit map held fixed. hit (total pixels), grad_rgb (total pixels x 3) and
+ * grad_depth (total pixels) are laid out as the batch's outputs (each view at its offset; views
+ * may differ in size). xyz (n x 3) and radii (n) are the centres and radii the forward marched,
+ * so an earlier forward can be differentiated after the cloud has moved; NULL takes the cloud's
+ * current ones. Accumulates into g_xyz (n x 3), g_r (n), g_rgb (n x 3), original point order:
+ * one sort of all views' hit pixels by point, then one reduction per point in pixel order
+ * (deterministic, no atomics). At most 2^31 pixels. */
+int rm_batch_render_adjoint(rm_batch* batch, const float* xyz, const float* radii,
+                            const int32_t* hit, const float* grad_rgb, const float* grad_depth,
+                            float* g_xyz, float* g_r, float* g_rgb);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,7 @@ SIGNATURES = {
     "rm_render_diff": (_i32, [_vp, _vp, C.POINTER(_f32), _f32, _i32, _i32, _vp, _vp, _vp]),
     "rm_render_adjoint": (_i32, [_vp, _vp, C.POINTER(_f32), _f32, _i32, _i32, _vp, _vp, _vp,
                                  _vp, _vp, _vp]),
+    "rm_batch_render_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

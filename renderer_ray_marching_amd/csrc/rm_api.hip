@@ -55,9 +55,11 @@ struct rm_tree {
     int32_t* inv = nullptr;                // leaf position of each original index (device)
     int32_t* level_order = nullptr;        // inner node ids grouped by level (device)
     std::vector<int32_t> level_start;      // host: offsets into level_order, nlevels + 1
+    uint64_t gen = 0;                      // bumped by every refit (which moves the points)
 };
 
 struct rm_cloud {
+    int device = 0;
     rm_tree* tree = nullptr;
     int64_t n = 0;
     float4* xyzr = nullptr;
@@ -65,6 +67,7 @@ struct rm_cloud {
     int32_t cull = 0;
     uint32_t* mask = nullptr;      // culling mask of single-view renders (allocated with culling)
     uint32_t* rmax = nullptr;      // device: bits of the largest radius
+    uint64_t gen = 0;              // the tree's refit generation this cloud's centres match
 };
 
 struct rm_batch {
@@ -173,8 +176,12 @@ DevView dev_view(const float* pose, float fov, int W, int H, int64_t pix_offset,
     return v;
 }
 
+// Makes the context's device current and clears the thread's last HIP error, so that the
+// launch checks of the call that follows (hipGetLastError after each launch) report its own
+// launches only, not a failure some other component of the process left behind.
 int set_device(rm_ctx* ctx) {
     RM_HIP(ctx, hipSetDevice(ctx->device));
+    (void)hipGetLastError();
     return RM_OK;
 }
 
@@ -232,6 +239,18 @@ int check_tree(rm_ctx* ctx, const rm_tree* t) {
 int check_cloud(rm_ctx* ctx, const rm_cloud* c) {
     if (!c) return fail(ctx, RM_ERR_INVALID, "null cloud");
     return check_tree(ctx, c->tree);
+}
+
+// A cloud marches its own centres over its tree's points; once the tree has been refit through
+// another cloud the two disagree (the warm start and the signed distance would read stale
+// centres), so the march refuses until this cloud is refit too.
+int check_fresh(rm_ctx* ctx, const rm_cloud* c, const char* where) {
+    if (int r = check_cloud(ctx, c)) return r;
+    if (c->gen != c->tree->gen)
+        return fail(ctx, RM_ERR_INVALID,
+                    std::string(where) + ": the cloud's tree was refit through another cloud; "
+                                         "refit this cloud (rm_cloud_refit) or create it again");
+    return RM_OK;
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -565,8 +584,10 @@ int rm_cloud_create(rm_ctx* ctx, const rm_tree* tree, const float* xyzw, const f
     if (int r = set_device(ctx)) return r;
     rm_cloud* c = new (std::nothrow) rm_cloud();
     if (!c) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");
+    c->device = tree->device;
     c->tree = const_cast<rm_tree*>(tree);
     c->n = n;
+    c->gen = tree->gen;
     if (hipMalloc(&c->xyzr, sizeof(float4) * (size_t)n) != hipSuccess ||
         hipMalloc(&c->colors, sizeof(float4) * (size_t)n) != hipSuccess ||
         hipMalloc(&c->rmax, sizeof(uint32_t)) != hipSuccess) {
@@ -586,7 +607,7 @@ int rm_cloud_create(rm_ctx* ctx, const rm_tree* tree, const float* xyzw, const f
 
 int rm_cloud_destroy(rm_cloud* c) {
     if (!c) return RM_OK;
-    if (c->tree) (void)hipSetDevice(c->tree->device);
+    (void)hipSetDevice(c->device);
     if (c->xyzr) (void)hipFree(c->xyzr);
     if (c->colors) (void)hipFree(c->colors);
     if (c->mask) (void)hipFree(c->mask);
@@ -640,6 +661,7 @@ int rm_cloud_refit(rm_ctx* ctx, rm_cloud* c, const float* xyz) {
     RM_HIP(ctx, rm::launch_refit(ctx->stream, t->nodes, t->pts, c->xyzr, xyz, t->n,
                                  t->level_order, t->level_start.data(),
                                  (int)t->level_start.size() - 1));
+    c->gen = ++t->gen;
     return RM_OK;
 }
 
@@ -652,7 +674,7 @@ int rm_render(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov, int 
     if (!c || !pose || !rgba) return fail(ctx, RM_ERR_INVALID, "rm_render: null input");
     if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render: empty image");
     if (!(fov > 0.0f && fov < 3.14159f)) return fail(ctx, RM_ERR_INVALID, "rm_render: bad fov");
-    if (int r = check_cloud(ctx, c)) return r;
+    if (int r = check_fresh(ctx, c, "rm_render")) return r;
     if (int r = set_device(ctx)) return r;
     DevView v = dev_view(pose, fov, W, H, 0, 0, 0);
     timing_begin(ctx);
@@ -791,7 +813,7 @@ int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
             !(v.fov_radians > 0.0f && v.fov_radians < 3.14159f))
             return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad view " + std::to_string(i));
     }
-    if (int r = check_cloud(ctx, cloud)) return r;
+    if (int r = check_fresh(ctx, cloud, "rm_batch_create")) return r;
     if (cloud->cull && nviews > 65535)
         return fail(ctx, RM_ERR_UNSUPPORTED,
                     "rm_batch_create: a culled batch holds at most 65535 views");
@@ -837,6 +859,7 @@ int rm_batch_get_tune(const rm_batch* b, rm_tune* tune) {
 int rm_batch_launch(rm_batch* b) {
     if (!b) return RM_ERR_INVALID;
     rm_ctx* ctx = b->ctx;
+    if (int r = check_fresh(ctx, b->cloud, "rm_batch_launch")) return r;
     if (int r = set_device(ctx)) return r;
     timing_begin(ctx);
     RM_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
@@ -850,6 +873,7 @@ int rm_batch_render_diff(rm_batch* b, float* rgb, float* depth) {
     if (!rgb || !depth) return fail(ctx, RM_ERR_INVALID, "rm_batch_render_diff: null output");
     if (!b->hit)
         return fail(ctx, RM_ERR_INVALID, "rm_batch_render_diff: the batch has no hit buffer");
+    if (int r = check_fresh(ctx, b->cloud, "rm_batch_render_diff")) return r;
     if (int r = set_device(ctx)) return r;
     DevCloud dc = dev_cloud(b->cloud);
     timing_begin(ctx);
@@ -865,10 +889,70 @@ int rm_batch_render_diff(rm_batch* b, float* rgb, float* depth) {
     return RM_OK;
 }
 
+int rm_batch_render_adjoint(rm_batch* b, const float* xyz, const float* radii,
+                            const int32_t* hit, const float* grad_rgb, const float* grad_depth,
+                            float* g_xyz, float* g_r, float* g_rgb) {
+    if (!b) return RM_ERR_INVALID;
+    rm_ctx* ctx = b->ctx;
+    if (!hit || !grad_rgb || !grad_depth || !g_xyz || !g_r || !g_rgb)
+        return fail(ctx, RM_ERR_INVALID, "rm_batch_render_adjoint: null input");
+    const rm_cloud* c = b->cloud;
+    if (int r = check_cloud(ctx, c)) return r;
+    if ((!xyz || !radii) && c->gen != c->tree->gen)
+        return fail(ctx, RM_ERR_INVALID,
+                    "rm_batch_render_adjoint: the cloud's tree was refit through another cloud; "
+                    "pass the forward's centres and radii");
+    if (b->laid_views != (int)b->views.size())
+        return fail(ctx, RM_ERR_INTERNAL, "rm_batch_render_adjoint: view table not laid out");
+    if (int r = set_device(ctx)) return r;
+    const int nv = (int)b->views.size();
+    std::vector<int64_t> cum((size_t)nv + 1);
+    int64_t P = 0;
+    for (int v = 0; v < nv; ++v) {
+        cum[v] = P;
+        P += (int64_t)b->views[v].width * b->views[v].height;
+    }
+    cum[nv] = P;
+    if (P > INT32_MAX)
+        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_batch_render_adjoint: more than 2^31 pixels");
+    int end_bit = 1;
+    while (end_bit < 32 && ((int64_t)1 << end_bit) <= c->n) ++end_bit;
+    size_t temp_bytes = 0;
+    RM_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (int32_t*)nullptr,
+                                                   (int32_t*)nullptr, (int32_t*)nullptr,
+                                                   (int32_t*)nullptr, (int)P, 0, end_bit,
+                                                   ctx->stream));
+    const size_t a = align_up(sizeof(int32_t) * (size_t)P);
+    const size_t tb = align_up(sizeof(int64_t) * cum.size());
+    if (int r = ensure_ws(ctx, 4 * a + tb + align_up(temp_bytes))) return r;
+    char* base = static_cast<char*>(ctx->ws);
+    int32_t* keys = reinterpret_cast<int32_t*>(base);
+    int32_t* vals = reinterpret_cast<int32_t*>(base + a);
+    int32_t* keys_s = reinterpret_cast<int32_t*>(base + 2 * a);
+    int32_t* vals_s = reinterpret_cast<int32_t*>(base + 3 * a);
+    int64_t* d_cum = reinterpret_cast<int64_t*>(base + 4 * a);
+    void* temp = base + 4 * a + tb;
+    RM_HIP(ctx, hipMemcpyAsync(d_cum, cum.data(), sizeof(int64_t) * cum.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    timing_begin(ctx);
+    RM_HIP(ctx, rm::launch_adjoint_batch_keys(ctx->stream, b->d_views, d_cum, nv, P, hit,
+                                              (int32_t)c->n, keys, vals));
+    RM_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_s, vals, vals_s,
+                                                   (int)P, 0, end_bit, ctx->stream));
+    RM_HIP(ctx, rm::launch_adjoint_batch_reduce(ctx->stream, dev_cloud(c), xyz, radii, b->d_views,
+                                                d_cum, nv, keys_s, vals_s, P, grad_rgb,
+                                                grad_depth, g_xyz, g_r, g_rgb));
+    timing_end(ctx);
+    // the copy's source is this function's vector: wait for it before returning
+    RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RM_OK;
+}
+
 int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
     if (!b) return RM_ERR_INVALID;
     rm_ctx* ctx = b->ctx;
     if (sample_stride < 1) sample_stride = 1;
+    if (int r = check_fresh(ctx, b->cloud, "rm_batch_autotune")) return r;
     if (int r = set_device(ctx)) return r;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1011,7 +1095,7 @@ int rm_render_batch_noalloc(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* v
             return fail(ctx, RM_ERR_INVALID,
                         "rm_render_batch_noalloc: bad view " + std::to_string(i));
     }
-    if (int r = check_cloud(ctx, cloud)) return r;
+    if (int r = check_fresh(ctx, cloud, "rm_render_batch_noalloc")) return r;
     if (cloud->cull)
         return fail(ctx, RM_ERR_UNSUPPORTED,
                     "rm_render_batch_noalloc: culling data grow with the cloud and views; "
@@ -1072,7 +1156,7 @@ int rm_render_diff(rm_ctx* ctx, const rm_cloud* c, const float* pose, float fov,
         return fail(ctx, RM_ERR_INVALID, "rm_render_diff: null input");
     if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render_diff: empty image");
     if (!(fov > 0.0f && fov < 3.14159f)) return fail(ctx, RM_ERR_INVALID, "rm_render_diff: bad fov");
-    if (int r = check_cloud(ctx, c)) return r;
+    if (int r = check_fresh(ctx, c, "rm_render_diff")) return r;
     if (int r = set_device(ctx)) return r;
     int64_t npix = (int64_t)W * H;
     if (int r = ensure_ws(ctx, align_up(sizeof(float4) * (size_t)npix))) return r;
@@ -1096,7 +1180,7 @@ int rm_render_adjoint(rm_ctx* ctx, const rm_cloud* c, const float* pose, float f
     if (W <= 0 || H <= 0) return fail(ctx, RM_ERR_INVALID, "rm_render_adjoint: empty image");
     if (!(fov > 0.0f && fov < 3.14159f))
         return fail(ctx, RM_ERR_INVALID, "rm_render_adjoint: bad fov");
-    if (int r = check_cloud(ctx, c)) return r;
+    if (int r = check_fresh(ctx, c, "rm_render_adjoint")) return r;
     if (int r = set_device(ctx)) return r;
     int64_t npix = (int64_t)W * H;
     if (npix > INT32_MAX) return fail(ctx, RM_ERR_UNSUPPORTED, "rm_render_adjoint: image too large");

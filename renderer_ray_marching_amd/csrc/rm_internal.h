@@ -142,6 +142,17 @@ hipError_t launch_adjoint_reduce(hipStream_t s, const DevCloud& c, const DevView
                                  int64_t npix, const float* grad_rgb, const float* grad_depth,
                                  float* g_xyz, float* g_r, float* g_rgb);
 
+// Adjoint over all views of a batch (pixels numbered view after view; cum: nviews + 1 entries).
+hipError_t launch_adjoint_batch_keys(hipStream_t s, const DevView* views, const int64_t* cum,
+                                     int nviews, int64_t P, const int32_t* hit, int32_t miss,
+                                     int32_t* keys, int32_t* vals);
+hipError_t launch_adjoint_batch_reduce(hipStream_t s, const DevCloud& c, const float* xyz,
+                                       const float* rad, const DevView* views, const int64_t* cum,
+                                       int nviews, const int32_t* keys_sorted,
+                                       const int32_t* vals_sorted, int64_t P,
+                                       const float* grad_rgb, const float* grad_depth,
+                                       float* g_xyz, float* g_r, float* g_rgb);
+
 // Scratch of the device tree build (all n-sized unless noted).
 struct BuildScratch {
     uint32_t* keys;

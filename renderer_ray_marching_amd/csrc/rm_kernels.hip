@@ -1204,6 +1204,86 @@ __global__ void adjoint_reduce_kernel(DevCloud c, DevView v, const int32_t* __re
     g_rgb[3 * (int64_t)key + 2] += c2;
 }
 
+// Adjoint over every view of a batch at once. The pixels of all views are numbered view after
+// view, j in [0, P); cum[v] is the number of the first pixel of view v (cum[nviews] = P), and
+// pixel j lives at views[v].pix_offset + (j - cum[v]) in the caller's concatenated arrays.
+__device__ __forceinline__ int view_of(const int64_t* __restrict__ cum, int nviews, int64_t j) {
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cum[mid] <= j)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void adjoint_batch_keys_kernel(const DevView* __restrict__ views,
+                                          const int64_t* __restrict__ cum, int nviews, int64_t P,
+                                          const int32_t* __restrict__ hit, int32_t miss,
+                                          int32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P) return;
+    const int v = view_of(cum, nviews, j);
+    const int h = hit[views[v].pix_offset + (j - cum[v])];
+    keys[j] = h >= 0 ? h : miss;
+    vals[j] = (int32_t)j;
+}
+
+// One thread per run of equal keys (one hit point): it sums the gradients of the run's pixels,
+// in pixel-number order, over all views, and adds them to the point. xyz (n x 3) and rad (n)
+// give the centres and radii the forward marched; null takes the cloud's current ones.
+__global__ void adjoint_batch_reduce_kernel(DevCloud c, const float* __restrict__ xyz,
+                                            const float* __restrict__ rad,
+                                            const DevView* __restrict__ views,
+                                            const int64_t* __restrict__ cum, int nviews,
+                                            const int32_t* __restrict__ keys,
+                                            const int32_t* __restrict__ vals, int64_t P,
+                                            const float* __restrict__ grad_rgb,
+                                            const float* __restrict__ grad_depth, float* g_xyz,
+                                            float* g_r, float* g_rgb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int key = keys[i];
+    if (key >= c.tree.n) return;
+    if (i > 0 && keys[i - 1] == key) return;
+    float4 cr = c.xyzr[key];
+    if (xyz) cr = make_float4(xyz[3 * (int64_t)key], xyz[3 * (int64_t)key + 1],
+                              xyz[3 * (int64_t)key + 2], cr.w);
+    if (rad) cr.w = rad[key];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, ar = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    for (int64_t k = i; k < P && keys[k] == key; ++k) {
+        const int64_t j = vals[k];
+        const int v = view_of(cum, nviews, j);
+        const DevView& dv = views[v];
+        const int64_t local = j - cum[v];
+        const int64_t pix = dv.pix_offset + local;
+        c0 += grad_rgb[3 * pix];
+        c1 += grad_rgb[3 * pix + 1];
+        c2 += grad_rgb[3 * pix + 2];
+        const float gd = grad_depth[pix];
+        if (gd == 0.0f) continue;
+        const int x = (int)(local % dv.W), y = (int)(local / dv.W);
+        const Ray r = make_ray([&](int u) { return dv.M[u]; }, dv.pane, dv.W, dv.H, x, y);
+        const SphereHit sh = sphere_entry(r, cr);
+        if (!sh.ok) continue;
+        // dt/dc = d - h / sqrt(disc), dt/dr = -r / sqrt(disc)
+        const float inv = 1.0f / sh.sq;
+        a0 += gd * (r.dx - sh.hx * inv);
+        a1 += gd * (r.dy - sh.hy * inv);
+        a2 += gd * (r.dz - sh.hz * inv);
+        ar += gd * (-cr.w * inv);
+    }
+    g_xyz[3 * (int64_t)key] += a0;
+    g_xyz[3 * (int64_t)key + 1] += a1;
+    g_xyz[3 * (int64_t)key + 2] += a2;
+    g_r[key] += ar;
+    g_rgb[3 * (int64_t)key] += c0;
+    g_rgb[3 * (int64_t)key + 1] += c1;
+    g_rgb[3 * (int64_t)key + 2] += c2;
+}
+
 // ------------------------------------------------------------------------------------------
 // Device tree build. The shape (segments, left counts, child refs) comes from the host; the
 // device keeps three lists of point ids, each sorted by one coordinate with ties by id, and per
@@ -1575,6 +1655,26 @@ hipError_t launch_adjoint_reduce(hipStream_t s, const DevCloud& c, const DevView
                                  float* g_xyz, float* g_r, float* g_rgb) {
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(grid_for(npix, 256)), dim3(256), 0, s, c, v,
                        keys_sorted, vals_sorted, npix, grad_rgb, grad_depth, g_xyz, g_r, g_rgb);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_batch_keys(hipStream_t s, const DevView* views, const int64_t* cum,
+                                     int nviews, int64_t P, const int32_t* hit, int32_t miss,
+                                     int32_t* keys, int32_t* vals) {
+    hipLaunchKernelGGL(adjoint_batch_keys_kernel, dim3(grid_for(P, 256)), dim3(256), 0, s, views,
+                       cum, nviews, P, hit, miss, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_batch_reduce(hipStream_t s, const DevCloud& c, const float* xyz,
+                                       const float* rad, const DevView* views, const int64_t* cum,
+                                       int nviews, const int32_t* keys_sorted,
+                                       const int32_t* vals_sorted, int64_t P,
+                                       const float* grad_rgb, const float* grad_depth,
+                                       float* g_xyz, float* g_r, float* g_rgb) {
+    hipLaunchKernelGGL(adjoint_batch_reduce_kernel, dim3(grid_for(P, 256)), dim3(256), 0, s, c,
+                       xyz, rad, views, cum, nviews, keys_sorted, vals_sorted, P, grad_rgb,
+                       grad_depth, g_xyz, g_r, g_rgb);
     return hipGetLastError();
 }
 

@@ -57,7 +57,14 @@ class DiffRenderer:
         self._render = ext.render
 
     def render(self, xyz: torch.Tensor, radii: torch.Tensor, rgb: torch.Tensor):
+        """Views of one size: img (V, H, W, 3), depth (V, H, W), hit (V, H, W). Views of mixed
+        sizes: the views concatenated, img (P, 3), depth (P,), hit (P,), with view v at
+        ``view_offsets()[v]``."""
         return self._render(xyz, radii, rgb, self.scene)
+
+    def view_offsets(self) -> list[int]:
+        """First pixel of each view in the concatenated outputs, then the total."""
+        return [int(x) for x in self.scene.view_offsets()]
 
 
 def refine(renderer: DiffRenderer, xyz, radii, rgb, target_rgb, target_depth, steps=200,

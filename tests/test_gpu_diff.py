@@ -312,3 +312,139 @@ def test_batched_forward_equals_per_view(rm_ctx, small_scene):
         assert rc_ == 0
         rm_ctx.sync()
         assert torch.equal(h1, hit[v]) and torch.equal(im1, img[v]) and torch.equal(d1, depth[v])
+
+
+def _per_view_forward_and_adjoint(rm_ctx, marcher, vs, g_img, g_dep, offs):
+    """rm_render_diff and rm_render_adjoint view by view (at the cloud's current parameters)."""
+    import ctypes as C
+    from renderer_ray_marching_amd import _abi
+    L = _abi.lib()
+    n = marcher.n
+    g = [torch.zeros((n, 3), device="cuda"), torch.zeros((n,), device="cuda"),
+         torch.zeros((n, 3), device="cuda")]
+    outs = []
+    for i, view in enumerate(vs):
+        W, H = view.width, view.height
+        im1 = torch.empty((H, W, 3), device="cuda")
+        d1 = torch.empty((H, W), device="cuda")
+        h1 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        pose = (C.c_float * 16)(*[float(x) for x in view.pose_colmajor()])
+        assert L.rm_render_diff(rm_ctx.h, marcher.h, pose, C.c_float(view.fov), W, H,
+                                C.c_void_p(im1.data_ptr()), C.c_void_p(d1.data_ptr()),
+                                C.c_void_p(h1.data_ptr())) == 0
+        a, b = offs[i], offs[i + 1]
+        gi = g_img[a:b].contiguous()
+        gd = g_dep[a:b].contiguous()
+        assert L.rm_render_adjoint(rm_ctx.h, marcher.h, pose, C.c_float(view.fov), W, H,
+                                   C.c_void_p(h1.data_ptr()), C.c_void_p(gi.data_ptr()),
+                                   C.c_void_p(gd.data_ptr()), *[C.c_void_p(t.data_ptr())
+                                                                for t in g]) == 0
+        outs.append((im1.reshape(-1, 3), d1.reshape(-1), h1.reshape(-1)))
+    rm_ctx.sync()
+    return outs, g
+
+
+def test_op_mixed_view_sizes_forward_and_backward(rm_ctx):
+    """The autograd op over views of three sizes: outputs concatenated view after view, each
+    equal to rm_render_diff of the view alone; the backward (one adjoint over all views) equals
+    the per-view adjoints summed, up to the order of the float sums."""
+    diff = _diff()
+    cloud = scenes.box_scene(30000, seed=14)
+    tree = ob.RefTree(cloud.xyzw(), 64)
+    _, rc, _ = tree.radii(0.05)
+    rc = np.maximum(rc, np.float32(0.01))
+    vs = [scenes.box_scene_view(cloud, w, h, f, distance=5.0, seed=s)
+          for (w, h, f, s) in ((96, 72, 80.0, 4), (33, 17, 30.0, 5), (64, 130, 70.0, 6))]
+    xyz = torch.from_numpy(cloud.xyz).cuda()
+    r = torch.from_numpy(rc).cuda()
+    rgb = torch.from_numpy(cloud.colors4()[:, :3].copy()).cuda()
+    R = diff.DiffRenderer(xyz, r, rgb, vs, ctx=rm_ctx)
+    offs = R.view_offsets()
+    assert offs[-1] == sum(v.width * v.height for v in vs)
+    x = xyz.clone().requires_grad_(True)
+    rr = r.clone().requires_grad_(True)
+    cc = rgb.clone().requires_grad_(True)
+    img, depth, hit = R.render(x, rr, cc)
+    assert img.shape == (offs[-1], 3) and depth.shape == (offs[-1],) and hit.shape == (offs[-1],)
+    rng = np.random.default_rng(3)
+    g_img = torch.from_numpy(rng.normal(size=(offs[-1], 3)).astype(np.float32)).cuda()
+    g_dep = torch.from_numpy(rng.normal(size=offs[-1]).astype(np.float32)).cuda()
+    ((img * g_img).sum() + (depth * g_dep).sum()).backward()
+    outs, g = _per_view_forward_and_adjoint(rm_ctx, R.marcher, vs, g_img, g_dep, offs)
+    for i, (im1, d1, h1) in enumerate(outs):
+        a, b = offs[i], offs[i + 1]
+        assert (h1 >= 0).any()
+        assert torch.equal(hit[a:b], h1) and torch.equal(img[a:b], im1)
+        assert torch.equal(depth[a:b], d1)
+    for got, ref in ((x.grad, g[0]), (rr.grad, g[1]), (cc.grad, g[2])):
+        assert torch.allclose(got, ref, rtol=1e-5, atol=1e-5 * float(ref.abs().max())), \
+            float((got - ref).abs().max())
+        assert float(ref.abs().max()) > 0
+
+
+def test_backward_of_an_earlier_forward(rm_ctx, small_scene):
+    """Two forwards at different centres, then the backward of the first: equal to a lone
+    forward + backward at the first centres (the backward uses the centres and radii its own
+    forward marched, not the cloud's latest)."""
+    cloud, rc, views, R, xyz, r, rgb = small_scene
+    rng = np.random.default_rng(7)
+    g_img = torch.from_numpy(rng.normal(size=(len(views), 72, 96, 3)).astype(np.float32)).cuda()
+    g_dep = torch.from_numpy(rng.normal(size=(len(views), 72, 96)).astype(np.float32)).cuda()
+
+    def fwd(x0, r0):
+        x = x0.clone().requires_grad_(True)
+        rr = r0.clone().requires_grad_(True)
+        cc = rgb.clone().requires_grad_(True)
+        img, depth, _ = R.render(x, rr, cc)
+        return (img * g_img).sum() + (depth * g_dep).sum(), (x, rr, cc)
+
+    moved = xyz + 0.003 * torch.randn(xyz.shape, device="cuda",
+                                      generator=torch.Generator("cuda").manual_seed(1))
+    loss_a, pa = fwd(xyz, r)
+    loss_b, pb = fwd(moved, r * 1.1)
+    loss_a.backward()
+    loss_b.backward()
+    loss_c, pc = fwd(xyz, r)
+    loss_c.backward()
+    for t_a, t_c in zip(pa, pc):
+        assert torch.equal(t_a.grad, t_c.grad)
+    assert not torch.equal(pa[0].grad, pb[0].grad)
+    # editing a saved input in place after its forward makes autograd refuse the backward
+    loss_d, pd = fwd(xyz, r)
+    with torch.no_grad():
+        pd[0].add_(0.001)
+    with pytest.raises(RuntimeError):
+        loss_d.backward()
+
+
+def test_refit_through_another_cloud_is_refused(rm_ctx):
+    """Two clouds on one tree: a refit through one leaves the other's centres stale, so the
+    other's march calls fail until it is refit too."""
+    from renderer_ray_marching_amd import _abi, api
+    from renderer_ray_marching_amd._abi import RMError
+    cloud = scenes.box_scene(5000, seed=2)
+    p = torch.from_numpy(cloud.xyzw()).cuda()
+    col = torch.from_numpy(cloud.colors4()).cuda()
+    t = api.KDTreeFlann(rm_ctx)
+    t.Build(p)
+    _, rc = t.radii(0.1)
+    a = api.PointcloudRayMarcher(p, col, rc, t)
+    b = api.PointcloudRayMarcher(p, col, rc, t)
+    v = scenes.box_scene_view(cloud, 40, 30, 30.0, 3.0, seed=1)
+    a.render_to_texture(v.pose_colmajor(), v.fov, 40, 30)
+    moved = (p[:, :3] + 0.001).contiguous()
+    a.refit(moved)
+    a.render_to_texture(v.pose_colmajor(), v.fov, 40, 30)
+    with pytest.raises(RMError) as e:
+        b.render_to_texture(v.pose_colmajor(), v.fov, 40, 30)
+    assert e.value.code == _abi.RM_ERR_INVALID and "refit" in str(e.value)
+    with pytest.raises(RMError):
+        api.Batch(b, [v])
+    b.refit(moved)
+    rgba_b, hit_b = b.render_to_texture(v.pose_colmajor(), v.fov, 40, 30)
+    with pytest.raises(RMError):                 # now a is the stale one
+        a.render_to_texture(v.pose_colmajor(), v.fov, 40, 30)
+    a.refit(moved)
+    rgba_a, hit_a = a.render_to_texture(v.pose_colmajor(), v.fov, 40, 30)
+    rm_ctx.sync()
+    assert torch.equal(hit_a, hit_b) and torch.equal(rgba_a, rgba_b)
