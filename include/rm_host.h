@@ -98,6 +98,15 @@ int rmh_lock_try(const char* lock_path, int* fd);
 /* Release the lock, close fd and remove the lock file. */
 int rmh_lock_release(int fd, const char* lock_path);
 
+/* ---- interactive viewer camera (no window) ------------------------------------------------ */
+/* The render poses and fields of view of `frames` frames of the viewer's FPS camera
+ * (src/camera.h:32-133, driven as main.cu:235-266 drives it) under a scripted input: one event
+ * per line, "<frame> key W|S|A|D|Q|E", "<frame> mouse <dx> <dy>" or "<frame> scroll <dy>"
+ * (script NULL: the built-in tour), with a fixed frame time dt (seconds). poses: frames x 16
+ * column-major camera-to-world floats (inverse of the view matrix); fovs: frames radians. The
+ * marcher.bin viewer renders exactly these poses. */
+int rmh_camera_poses(const char* script, int frames, float dt, float* poses, float* fovs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 
 #include "rm_host.h"
 #include "rm_io.h"
+#include "viewer.h"
 
 namespace {
 
@@ -166,6 +167,21 @@ int rmh_lock_try(const char* lock_path, int* fd) {
 int rmh_lock_release(int fd, const char* lock_path) {
     if (!lock_path || fd < 0) return fail(RMH_ERR_INVALID, "rmh_lock_release: bad argument");
     return guarded([&] { rmh::lock_release(fd, lock_path); });
+}
+
+int rmh_camera_poses(const char* script, int frames, float dt, float* poses, float* fovs) {
+    if (frames < 0 || (frames > 0 && (!poses || !fovs)))
+        return fail(RMH_ERR_INVALID, "rmh_camera_poses: bad arguments");
+    return guarded([&] {
+        const std::vector<rmh::InputEvent> ev =
+            script ? rmh::parse_input_script(script) : rmh::default_tour();
+        rmh::FpsCamera cam;
+        for (int f = 0; f < frames; ++f) {
+            cam.pose(poses + 16 * (size_t)f);
+            fovs[f] = cam.zoom;
+            rmh::apply_frame_events(cam, ev, f, dt);
+        }
+    });
 }
 
 }  // extern "C"

@@ -15,8 +15,12 @@
 //     otherwise, and with -s equal to the point count sets every radius to the first one);
 //   * matrices with fewer than 4 x 4 numbers are rejected (glm::from_json reads past them);
 //   * a render key with a single path component writes into <output> itself;
-//   * without -d there is no window to open: the radii are computed and cached, then the
-//     program reports that the interactive viewer is not part of this build.
+//   * without -d there is no window to open: the viewer's compute half runs instead. Its FPS
+//     camera (src/camera.h) is driven by a scripted input (--script, or a built-in tour), every
+//     frame is rendered at 1024 x 768 (src/common.h:13-14) into a device buffer with rm_render
+//     at inverse(view) and the camera's zoom as field of view (main.cu:249-258), and "<n> FPS"
+//     is printed once a second as FPSCounter::update does (src/utils.cu:104-112); --dump
+//     writes frames and their poses for inspection.
 #include <fcntl.h>
 #include <hip/hip_runtime_api.h>
 #include <unistd.h>
@@ -31,6 +35,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <iterator>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -39,6 +44,7 @@
 
 #include "rm.h"
 #include "rm_io.h"
+#include "viewer.h"
 
 namespace fs = std::filesystem;
 using std::cout;
@@ -56,6 +62,11 @@ struct Args {
     bool headless = false, precompute = false, ignore_existing = false;
     int batch = 1;
     bool plain = false;   // -b 0: the plain per-view kernel (rm_render)
+    // viewer (without -d)
+    string script_path, dump_dir;
+    int frames = -1;      // -1: as many as the script covers (120 for the built-in tour)
+    float dt = -1.0f;     // fixed frame time in seconds; < 0: the wall clock, as the reference
+    int dump_every = 0;   // with --dump: every k-th frame (0: only the last)
 };
 
 // CLI11 exit codes for the errors this parser reports.
@@ -78,7 +89,15 @@ const char* kHelp =
     "  -p,--precompute-radii       Precompute radii even if already precomputed.\n"
     "  -i,--ignore_existing        Ignore existing renders and forcefully rewrite them.\n"
     "  -b,--batch INT              Views marched per batched GPU launch (default 1: one launch per view;\n"
-    "                              0: the plain per-view kernel).\n";
+    "                              0: the plain per-view kernel).\n"
+    "Without -d (no window in this build: the viewer renders a scripted camera path):\n"
+    "  --script TEXT               Input events, one per line: '<frame> key W|S|A|D|Q|E',\n"
+    "                              '<frame> mouse <dx> <dy>', '<frame> scroll <dy>' (default: a built-in tour).\n"
+    "  --frames INT                Frames to render (default: to the script's last event).\n"
+    "  --dt FLOAT                  Fixed frame time in seconds (default: the wall clock).\n"
+    "  --dump TEXT                 Write frames (frame_NNNNN.npy, H x 4W float32, bottom-up) and\n"
+    "                              their poses (frame_NNNNN.pose: 16 column-major floats, then the fov).\n"
+    "  --dump-every INT            With --dump: every k-th frame (default: only the last).\n";
 
 struct Opt {
     char short_name;
@@ -100,8 +119,16 @@ int parse_args(int argc, char** argv, Args& a) {
         {'p', "precompute-radii", 0, &a.precompute},
         {'i', "ignore_existing", 0, &a.ignore_existing},
         {'b', "batch", 2, &a.batch},
+        {'\0', "script", 1, &a.script_path},
+        {'\0', "frames", 2, &a.frames},
+        {'\0', "dt", 3, &a.dt},
+        {'\0', "dump", 1, &a.dump_dir},
+        {'\0', "dump-every", 2, &a.dump_every},
     };
-    auto label = [](const Opt& o) { return string("-") + o.short_name + ",--" + o.long_name; };
+    auto label = [](const Opt& o) {
+        return o.short_name ? string("-") + o.short_name + ",--" + o.long_name
+                            : string("--") + o.long_name;
+    };
     auto assign = [&](Opt& o, const string& v) -> int {
         if (o.kind != 0 && o.seen) {
             std::cerr << label(o) << ": at most 1 value allowed" << endl;
@@ -173,8 +200,9 @@ int parse_args(int argc, char** argv, Args& a) {
         }
         if (s.size() > 1 && s[0] == '-') {
             for (size_t j = 1; j < s.size(); ++j) {
-                auto it = std::find_if(opts.begin(), opts.end(),
-                                       [&](const Opt& o) { return s[j] == o.short_name; });
+                auto it = std::find_if(opts.begin(), opts.end(), [&](const Opt& o) {
+                    return o.short_name && s[j] == o.short_name;
+                });
                 if (it == opts.end()) {
                     extras.push_back(s);
                     break;
@@ -211,6 +239,10 @@ int parse_args(int argc, char** argv, Args& a) {
     if (a.batch == 0) {
         a.plain = true;
         a.batch = 1;
+    }
+    if (a.frames < -1 || a.dump_every < 0) {
+        std::cerr << "--frames and --dump-every must not be negative" << endl;
+        return kConversionError;
     }
     return -1;
 }
@@ -408,6 +440,70 @@ private:
     std::vector<float> host_;
 };
 
+// ---- the viewer without a window ---------------------------------------------------------
+
+// The render loop of main.cu:235-266 with scripted input: per frame, the frame time, the render
+// of the camera's pose at 1024 x 768 (the reference's RGBA32F texture, here a device buffer),
+// the FPS count, then the frame's input events.
+void run_viewer(const Args& a, Gpu& gpu, rm_cloud* cloud) {
+    constexpr int kW = 1024, kH = 768;   // SCREEN_WIDTH, SCREEN_HEIGHT (src/common.h:13-14)
+    std::vector<rmh::InputEvent> events;
+    if (!a.script_path.empty()) {
+        std::ifstream in(a.script_path);
+        if (!in) throw std::runtime_error("cannot read input script '" + a.script_path + "'");
+        std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        events = rmh::parse_input_script(text);
+    } else {
+        events = rmh::default_tour();
+    }
+    int frames = a.frames;
+    if (frames < 0) {
+        frames = 0;
+        for (const auto& e : events) frames = std::max(frames, e.frame + 1);
+    }
+    if (!a.dump_dir.empty()) fs::create_directories(a.dump_dir);
+    DevBuf rgba((size_t)kW * kH * 16), hit((size_t)kW * kH * 4);
+    std::vector<float> host;
+    rmh::FpsCamera cam;
+    rmh::FpsCounter fps;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto seconds = [&] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    float last_frame = 0.0f;
+    for (int f = 0; f < frames; ++f) {
+        const float current = (float)seconds();
+        const float dt = a.dt >= 0.0f ? a.dt : current - last_frame;
+        last_frame = current;
+        float pose[16];
+        cam.pose(pose);
+        const float fov = cam.zoom;
+        gpu.check(rm_render(gpu.ctx, cloud, pose, fov, kW, kH, rgba.as<float>(),
+                            hit.as<int32_t>()), "rm_render");
+        gpu.check(rm_ctx_synchronize(gpu.ctx), "rm_render");
+        const bool dump = !a.dump_dir.empty() &&
+                          (a.dump_every > 0 ? f % a.dump_every == 0 : f == frames - 1);
+        if (dump) {
+            host.resize((size_t)kW * kH * 4);
+            to_host(host.data(), rgba.p, host.size() * sizeof(float));
+            char name[32];
+            std::snprintf(name, sizeof(name), "frame_%05d", f);
+            const string base = (fs::path(a.dump_dir) / name).string();
+            rmh::write_npy_f32(base + ".npy", host.data(), kH, 4 * kW);
+            std::ofstream pf(base + ".pose");
+            pf.precision(9);
+            for (int i = 0; i < 16; ++i) pf << pose[i] << (i == 15 ? "\n" : " ");
+            pf << fov << "\n";
+            if (!pf) throw std::runtime_error("cannot write " + base + ".pose");
+        }
+        std::string line;
+        if (fps.update(seconds(), &line)) cout << line << endl;
+        rmh::apply_frame_events(cam, events, f, dt);
+    }
+    cout << "Rendered " << frames << " frames at " << kW << "x" << kH << " in " << seconds()
+         << " s" << endl;
+}
+
 int run(const Args& a) {
     rmh::PlyCloud pc = rmh::load_ply(a.pcd_path);
     const int64_t n_full = pc.n;
@@ -502,10 +598,7 @@ int run(const Args& a) {
                 cout << "The matrix file '" << a.matrix_path << "' was not found." << endl;
             }
         } else {
-            std::cerr << "The interactive viewer is not part of this build; run with -d (headless) "
-                         "and -m <matrices>."
-                      << endl;
-            successful_run = false;
+            run_viewer(a, gpu, cloud);
         }
     } catch (const std::exception& e) {
         std::cerr << e.what() << endl;

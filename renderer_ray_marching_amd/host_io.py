@@ -49,6 +49,7 @@ SIGNATURES = {
     "rmh_view_paths": (_i32, [_cp, _cp, _cp, _cp, _cp, C.c_size_t]),
     "rmh_lock_try": (_i32, [_cp, C.POINTER(_i32)]),
     "rmh_lock_release": (_i32, [_i32, _cp]),
+    "rmh_camera_poses": (_i32, [_cp, _i32, C.c_float, _vp, _vp]),
 }
 
 _lib = None
@@ -180,3 +181,14 @@ def lock_try(lock_path):
 
 def lock_release(fd, lock_path):
     _check(lib().rmh_lock_release(fd, _b(lock_path)))
+
+
+def camera_poses(frames: int, dt: float, script: str | None = None):
+    """Render poses (frames, 16) column-major and fields of view (frames,) of the viewer's FPS
+    camera under a scripted input (None: the built-in tour), fixed frame time dt."""
+    poses = np.empty((frames, 16), np.float32)
+    fovs = np.empty(frames, np.float32)
+    _check(lib().rmh_camera_poses(None if script is None else script.encode(), frames, float(dt),
+                                  poses.ctypes.data_as(C.c_void_p),
+                                  fovs.ctypes.data_as(C.c_void_p)))
+    return poses, fovs

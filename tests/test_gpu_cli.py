@@ -196,3 +196,51 @@ def test_cli_cache_errors(scene, tmp_path):
     bad.write_text(json.dumps({"train": {}}))
     r = _run(["-d", "-f", str(ply), "-m", str(bad), "-o", str(tmp_path / "o")], tmp_path)
     assert r.returncode == 1 and "val" in r.stderr
+
+
+def test_viewer_renders_the_scripted_path_and_counts_fps(rm_ctx, tmp_path):
+    """Without -d: the viewer's compute half (main.cu:235-266 without the window). A box scene
+    centred on the origin (the camera starts at (0, 0, 3) looking down -z), a scripted path of
+    1500 frames at 1024x768: every dumped frame equals rm_render of the dumped pose and field of
+    view on every pixel, the poses are the camera's (rmh_camera_poses of the same script), and
+    "<n> FPS" is printed once per second of the run."""
+    import re
+    import torch
+    from renderer_ray_marching_amd import api
+    cloud = scenes.box_scene(30_000, seed=8)
+    xyz = (cloud.xyz - cloud.xyz.mean(0)).astype(np.float32)
+    ply = tmp_path / "c.ply"
+    hio.write_ply(ply, xyz, cloud.rgb)
+    script = "\n".join([f"{f} mouse {3 if (f // 150) % 2 == 0 else -3} 0" for f in range(0, 1500, 10)]
+                       + [f"{f} mouse 0 {2 if f < 700 else -2}" for f in range(600, 800, 20)]
+                       + [f"{f} scroll 0.5" for f in range(1000, 1040)])
+    (tmp_path / "path.txt").write_text(script + "\n")
+    dump = tmp_path / "frames"
+    r = _run(["-f", str(ply), "-r", "0.1", "--script", str(tmp_path / "path.txt"), "--frames",
+              "1500", "--dt", "0.01", "--dump", str(dump), "--dump-every", "250"], tmp_path)
+    assert r.returncode == 0
+    lines = r.stdout.splitlines()
+    fps = [int(x.split()[0]) for x in lines if re.fullmatch(r"\d+ FPS", x)]
+    secs = float(re.search(r"Rendered 1500 frames at 1024x768 in ([0-9.]+) s", r.stdout).group(1))
+    assert int(secs) - 1 <= len(fps) <= int(secs) and sum(fps) <= 1500
+    poses, fovs = hio.camera_poses(1500, 0.01, script)
+    # the same cloud and radii through the library
+    xyzw, rgba = hio.load_ply(ply)
+    radii = np.minimum(hio.read_radii(str(ply) + ".kdtree.radii"), np.float32(0.1))
+    ctx = rm_ctx
+    t = api.KDTreeFlann(ctx)
+    p = torch.from_numpy(xyzw).cuda()
+    t.Build(p)
+    m = api.PointcloudRayMarcher(p, torch.from_numpy(rgba).cuda(), torch.from_numpy(radii).cuda(),
+                                 t)
+    hits = 0
+    for f in range(0, 1500, 250):
+        vals = np.array(open(dump / f"frame_{f:05d}.pose").read().split(), np.float32)
+        pose, fov = vals[:16], vals[16]
+        assert np.array_equal(pose, poses[f]) and fov == fovs[f], f
+        frame = np.load(dump / f"frame_{f:05d}.npy").reshape(768, 1024, 4)
+        ref, hit = m.render_to_texture(pose, float(fov), 1024, 768)
+        ctx.sync()
+        assert np.array_equal(ref.cpu().numpy(), frame), f
+        hits += int((hit >= 0).sum().item())
+    assert hits > 100_000
