@@ -6,7 +6,7 @@
 // report or a crash fails the test. It also writes PNG / depth / .npy outputs and exercises
 // view path and lock helpers on valid and edge-case arguments.
 //
-// usage: host_io_fuzz <kind> <file> <scratch-dir>     kind: ply | radii | matrices | io
+// usage: host_io_fuzz <kind> <file> <scratch-dir>     kind: ply | radii | matrices | script | io
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -69,6 +69,12 @@ int parse(const std::string& kind, const std::string& path) {
             rmh_matrices_free(m);
         }
         return st;
+    }
+    if (kind == "script") {
+        std::vector<char> b = slurp(path);
+        b.push_back('\0');
+        float poses[16 * 8], fovs[8];
+        return rmh_camera_poses(b.data(), 8, 0.016f, poses, fovs);
     }
     return -100;
 }
@@ -145,7 +151,7 @@ int io(const std::string& dir) {
 
 int main(int argc, char** argv) {
     if (argc != 4) {
-        std::fprintf(stderr, "usage: %s ply|radii|matrices|io <file> <scratch-dir>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s ply|radii|matrices|script|io <file> <scratch-dir>\n", argv[0]);
         return 1;
     }
     const std::string kind = argv[1];

@@ -1,6 +1,7 @@
 """Host drop-in I/O and the CPU oracle under AddressSanitizer + UndefinedBehaviorSanitizer.
 
-The parsers of librm_host (PLY, the .kdtree.radii text archive, the matrices JSON) and its
+The parsers of librm_host (PLY, the .kdtree.radii text archive, the matrices JSON, the viewer's
+input script) and its
 writers are compiled with g++ -fsanitize=address,undefined together with
 tests/sanitize/host_io_fuzz.cpp, which feeds every parser its valid input and thousands of
 deterministic corruptions of it (truncations, byte changes). Each must be parsed or rejected with
@@ -19,7 +20,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "renderer_ray_marching_amd", "host")
-SRC = ["json.cpp", "ply.cpp", "radii_cache.cpp", "image_io.cpp", "views.cpp", "capi.cpp"]
+SRC = ["json.cpp", "ply.cpp", "radii_cache.cpp", "image_io.cpp", "views.cpp", "viewer.cpp",
+       "capi.cpp"]
 
 pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
 
@@ -92,6 +94,12 @@ def test_matrices_parser_under_sanitizers(fuzzer, tmp_path):
            "val": {"s/b/y_color.png": {"camera_pose": pose, "calibration_mat": calib}}}
     (tmp_path / "m.json").write_text(json.dumps(doc))
     assert "corrupted inputs" in _run(fuzzer, "matrices", tmp_path / "m.json", tmp_path)
+
+
+def test_viewer_script_parser_under_sanitizers(fuzzer, tmp_path):
+    (tmp_path / "s.txt").write_text("# a path\n0 key W\n0 mouse 12.5 -3\n1 key A\n2 scroll 1\n"
+                                    "3 key E  # up\n5 mouse 0 900\n7 scroll -2.5\n")
+    assert "corrupted inputs" in _run(fuzzer, "script", tmp_path / "s.txt", tmp_path)
 
 
 def test_writers_and_paths_under_sanitizers(fuzzer, tmp_path):
