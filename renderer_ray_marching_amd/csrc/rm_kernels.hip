@@ -191,10 +191,33 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
     }
 }
 
-template <int K, bool NODUP, bool CULL = false>
+// WIDE: a leaf of exactly kWideLeaf points (every leaf of a leaf-16 tree but the last one) is
+// loaded whole before any of it is scored, so its four 64-byte scalar loads are in flight
+// together instead of one after another.
+constexpr int kWideLeaf = 16;
+
+template <int K, bool NODUP, bool CULL = false, bool WIDE = false>
 __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, float qx, float qy,
                                                  float qz, TopK<K>& tk, CUint* mask = nullptr) {
     CFloat* p = pf + (int64_t)b * 4;
+    if (WIDE && !CULL && cnt == kWideLeaf) {
+        float x[kWideLeaf], y[kWideLeaf], z[kWideLeaf];
+        uint32_t id[kWideLeaf];
+#pragma unroll
+        for (int u = 0; u < kWideLeaf; ++u) {
+            x[u] = p[4 * u + 0];
+            y[u] = p[4 * u + 1];
+            z[u] = p[4 * u + 2];
+            id[u] = __float_as_uint(p[4 * u + 3]);
+        }
+#pragma unroll
+        for (int u = 0; u < kWideLeaf; ++u) {
+            const float d = dist2(x[u], y[u], z[u], qx, qy, qz);
+            if (__builtin_amdgcn_ballot_w64(d <= tk.worst_d2()) != 0)
+                tk.template add<NODUP>(make_key(d, id[u]));
+        }
+        return;
+    }
     int j = 0;
     for (; j + 4 <= cnt; j += 4) {
         // culled points (wave-uniform bits) are skipped by the scalar unit
@@ -255,7 +278,7 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
             if ((__builtin_amdgcn_ballot_w64(md <= tk.worst_d2()) & amask) == 0) continue;
             const int b = fi[12 + s];
             if (b >= skip_b && b < skip_e) continue;
-            packet_scan_leaf<K, NODUP>(pf, b, fi[14 + s], qx, qy, qz, tk);
+            packet_scan_leaf<K, NODUP, false, true>(pf, b, fi[14 + s], qx, qy, qz, tk);
         } else {
             packet_push_children<K>(nf, e >> 1, qx, qy, qz, amask, tk, wstk, sp, err);
         }
@@ -496,7 +519,12 @@ __global__ __launch_bounds__(WPB * 64) void radii_kernel(DevTree t, float* __res
     // leaves when leaf_max divides 64 (all leaves hold leaf_max points except the last one);
     // otherwise there is no pre-scan.
     const int gcnt = (64 % t.leaf_max == 0) ? (int)min((int64_t)64, t.n - g0) : 0;
-    packet_scan_leaf<2, false>((CFloat*)t.pts, (int)g0, gcnt, q.x, q.y, q.z, tk);
+    // in chunks of kWideLeaf points, each loaded whole by scalar loads (g0 is wave-uniform)
+    const int gb = __builtin_amdgcn_readfirstlane((int)g0);
+    const int gc = __builtin_amdgcn_readfirstlane(gcnt);
+    for (int c = 0; c < gc; c += kWideLeaf)
+        packet_scan_leaf<2, false, false, true>((CFloat*)t.pts, gb + c, min(kWideLeaf, gc - c),
+                                                q.x, q.y, q.z, tk);
     knn_packet<2>(t, q.x, q.y, q.z, active, tk, wstk[w], err, (int)g0, (int)(g0 + gcnt));
     if (active) r_leaf[pos] = sqrtf(key_d2(tk.key[1]));
 }
