@@ -148,14 +148,23 @@ typedef struct rm_view {
 typedef struct rm_tune {
     int32_t tile_w, tile_h;   /* pixels per claimed tile: tile_w * tile_h, a multiple of 64
                                  (<= 4096); a wavefront keeps 64 rays of its tile in flight */
-    int32_t mode;             /* 0 = packet traversal (wave-uniform), 1 = per-lane traversal */
+    int32_t mode;             /* packet size: 0 = 64-ray packets (a wavefront's rays traverse
+                                 the tree together, wave-uniform stack), 1 = 1-ray packets
+                                 (every lane traverses alone with its own stack) */
     int32_t warm;             /* 1 = warm-start bound from the previous step's neighbours */
     int32_t waves_per_block;  /* 1, 2 or 4 */
     int32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu (0 = 32 / waves_per_block,
                                  8 wavefronts per SIMD) */
     int32_t refill;           /* lanes whose ray ended take the tile's next pixels once at
                                  least this many (1..64) are idle */
+    int32_t stack_depth;      /* mode 1: entries of each lane's traversal stack kept in LDS
+                                 (1..RM_MAX_STACK_DEPTH), the deeper ones going to a global
+                                 area the batch allocates; 0, or at least the tree's need
+                                 (depth + 3), keeps the whole stack in LDS. Fewer LDS entries
+                                 leave room for more resident wavefronts. Mode 0 ignores it.
+                                 Results are identical for every value. */
 } rm_tune;
+#define RM_MAX_STACK_DEPTH 64
 
 /* A batch of views marched by one persistent kernel over a shared tile queue, launched as a
  * captured HIP graph. rgba / hit (hit may be NULL) are the concatenated outputs, sized by the

@@ -86,6 +86,8 @@ struct rm_batch {
     int blocks = 0;
     int32_t cull = 0;              // the cloud's culling setting at creation
     uint32_t* d_mask = nullptr;    // culling masks, one per view
+    int* d_ovf = nullptr;          // per-lane stack overflow area (tune.stack_depth), grow-only
+    int64_t ovf_ints = 0;
 };
 
 namespace {
@@ -699,6 +701,7 @@ rm_tune default_tune() {
     t.waves_per_block = 2;
     t.blocks_per_cu = 0;
     t.refill = 1;
+    t.stack_depth = 0;
     return t;
 }
 
@@ -708,7 +711,8 @@ bool tune_valid(const rm_tune& t) {
            (t.mode == 0 || t.mode == 1) &&
            (t.warm == 0 || t.warm == 1) &&
            (t.waves_per_block == 1 || t.waves_per_block == 2 || t.waves_per_block == 4) &&
-           t.blocks_per_cu >= 0 && t.blocks_per_cu <= 64 && t.refill >= 1 && t.refill <= 64;
+           t.blocks_per_cu >= 0 && t.blocks_per_cu <= 64 && t.refill >= 1 && t.refill <= 64 &&
+           t.stack_depth >= 0 && t.stack_depth <= RM_MAX_STACK_DEPTH;
 }
 
 // Lay out the views' tiles for a tune and upload the view table.
@@ -738,6 +742,19 @@ int batch_layout(rm_batch* b, int view_stride = 1) {
         bpc = std::max(1, 32 / b->tune.waves_per_block);
     }
     b->blocks = ctx->num_cus * bpc;
+    // per-lane traversal with part of the stack outside LDS: one column of overflow entries
+    // per thread of the persistent grid
+    if (b->tune.mode == 1) {
+        const int64_t need = rm::lane_ovf_ints(b->cloud->tree->depth, b->tune.stack_depth,
+                                               (int64_t)b->blocks * b->tune.waves_per_block * 64);
+        if (need > b->ovf_ints) {
+            if (b->d_ovf) (void)hipFree(b->d_ovf);
+            b->d_ovf = nullptr;
+            b->ovf_ints = 0;
+            RM_HIP(ctx, hipMalloc(&b->d_ovf, (size_t)need * sizeof(int)));
+            b->ovf_ints = need;
+        }
+    }
     return RM_OK;
 }
 
@@ -757,6 +774,8 @@ rm::BatchParams batch_params(rm_batch* b) {
     p.err = b->ctx->d_err;
     p.out.rgba = reinterpret_cast<float4*>(b->rgba);
     p.out.hit = b->hit;
+    p.lane_lds = b->tune.stack_depth;
+    p.lane_ovf = b->d_ovf;
     return p;
 }
 
@@ -989,10 +1008,10 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         r = measure(t, ms);
         if (r == RM_OK && verbose)
             std::fprintf(stderr,
-                         "[rm autotune] tile %dx%d mode %d warm %d wpb %d bpc %d refill %d: "
-                         "%.3f ms\n",
+                         "[rm autotune] tile %dx%d mode %d warm %d wpb %d bpc %d refill %d "
+                         "stack %d: %.3f ms\n",
                          t.tile_w, t.tile_h, t.mode, t.warm, t.waves_per_block, t.blocks_per_cu,
-                         t.refill, ms);
+                         t.refill, t.stack_depth, ms);
         // a candidate must beat the incumbent by 2 % (timing noise would otherwise flip
         // between equivalent settings from run to run)
         if (r == RM_OK && ms < best_ms * 0.98f) {
@@ -1000,16 +1019,23 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
             best = t;
         }
     };
-    // coordinate descent: traversal mode x warm start, then tile shape and size (tiles above
-    // 64 pixels refill lanes locally), then block shape, then the refill threshold
+    // coordinate descent: packet size (a wavefront's 64 rays traverse together, or every ray
+    // alone) x warm start x stack depth (per-lane stacks: entries kept in LDS, the rest in
+    // global memory; fewer in LDS leave room for more resident wavefronts), then tile shape and
+    // size (tiles above 64 pixels refill lanes locally), then block shape, then the refill
+    // threshold
     rm_tune base = default_tune();
+    const int need = rm::lane_stack_need(b->cloud->tree->depth);
     for (int mode = 0; mode < 2; ++mode)
-        for (int warm = 0; warm < 2; ++warm) {
-            rm_tune t = base;
-            t.mode = mode;
-            t.warm = warm;
-            consider(t);
-        }
+        for (int warm = 0; warm < 2; ++warm)
+            for (int depth : {0, 16, 8}) {
+                if (depth != 0 && (mode == 0 || depth >= need)) continue;
+                rm_tune t = base;
+                t.mode = mode;
+                t.warm = warm;
+                t.stack_depth = depth;
+                consider(t);
+            }
     const int shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {16, 8}};
     rm_tune cur = best;
     for (auto& sh : shapes) {
@@ -1064,6 +1090,7 @@ int rm_batch_destroy(rm_batch* b) {
     if (b->d_views) (void)hipFree(b->d_views);
     if (b->d_queue) (void)hipFree(b->d_queue);
     if (b->d_mask) (void)hipFree(b->d_mask);
+    if (b->d_ovf) (void)hipFree(b->d_ovf);
     delete b;
     return RM_OK;
 }
@@ -1138,6 +1165,8 @@ int rm_render_batch_noalloc(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* v
     p.err = ctx->d_err;
     p.out.rgba = reinterpret_cast<float4*>(rgba);
     p.out.hit = hit;
+    p.lane_lds = 0;          // the default tune keeps any per-lane stack whole in LDS
+    p.lane_ovf = nullptr;
     timing_begin(ctx);
     RM_HIP(ctx, rm::launch_reset_queue(ctx->stream, p.queue));
     RM_HIP(ctx, rm::launch_batch(ctx->stream, p, t.mode, t.warm, t.waves_per_block,

@@ -88,7 +88,22 @@ struct BatchParams {
     MarchOut out;
     const uint32_t* cull_mask;  // cloud.cull: the culling data of each view, cull_words apart
     int64_t cull_words;
+    // per-lane traversal: stack entries kept in LDS per lane (0: the whole stack), and the
+    // global area for the deeper ones, lane_ovf_words ints per thread of the grid (column
+    // layout); null when the whole stack is in LDS
+    int32_t lane_lds;
+    int* lane_ovf;
 };
+
+// Per-lane stack entries a traversal of a tree of this depth needs (root children, then at most
+// two per level below).
+inline int lane_stack_need(int tree_depth) { return tree_depth + 3; }
+// Ints of per-lane overflow area a grid of `threads` threads needs for a tree of this depth when
+// `lane_lds` entries per lane are in LDS (0 when they cover the whole stack).
+inline int64_t lane_ovf_ints(int tree_depth, int lane_lds, int64_t threads) {
+    const int need = lane_stack_need(tree_depth);
+    return (lane_lds <= 0 || lane_lds >= need) ? 0 : (int64_t)(need - lane_lds) * threads;
+}
 
 // Culling data of one view: a point mask (bit i: point i of the leaf order is inside the view
 // frustum; 32 points a word, whole pairs of words plus a spare pair), then one byte per inner

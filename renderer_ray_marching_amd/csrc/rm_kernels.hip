@@ -344,11 +344,46 @@ __device__ __forceinline__ NodeRec load_node(const RmNode* nodes, int ni) {
     return n;
 }
 
-template <bool CULL = false, class Set>
+// A lane's stack: entry i lives in LDS at lds[i * stride] while i < lds_depth, deeper entries
+// in global memory at ovf[(i - lds_depth) * ostride + ocol] (column ocol of a row-major area
+// with one column per thread of the grid; the area holds fewer than 2^31 ints). The column is
+// obase + (lds - lds0), the lane's offset in its block's LDS stack array, so that no register
+// beyond the LDS pointer is kept per lane. OVF = false: the whole stack (maxsp entries) is in
+// LDS and ovf is not used.
+struct LaneStack {
+    int* lds;
+    int stride;
+    int maxsp;
+    int lds_depth;
+    int* ovf;
+    const int* lds0;
+    uint32_t obase, ostride;
+};
+
+template <bool OVF>
+__device__ __forceinline__ uint32_t lane_ovf_at(const LaneStack& ls, int i) {
+    return (uint32_t)(i - ls.lds_depth) * ls.ostride + ls.obase + (uint32_t)(ls.lds - ls.lds0);
+}
+
+template <bool OVF>
+__device__ __forceinline__ void lane_put(const LaneStack& ls, int i, int v) {
+    if (!OVF || i < ls.lds_depth)
+        ls.lds[i * ls.stride] = v;
+    else
+        ls.ovf[lane_ovf_at<OVF>(ls, i)] = v;
+}
+
+template <bool OVF>
+__device__ __forceinline__ int lane_get(const LaneStack& ls, int i) {
+    if (!OVF || i < ls.lds_depth) return ls.lds[i * ls.stride];
+    return ls.ovf[lane_ovf_at<OVF>(ls, i)];
+}
+
+template <bool CULL = false, bool OVF = false, class Set>
 __device__ __forceinline__ void lane_push_children(const RmNode* nodes, int ni, float qx,
                                                    float qy, float qz, const Set& tk,
-                                                   int* lstk, int stride, int& sp, int maxsp,
-                                                   int* err, const uint8_t* nbits = nullptr) {
+                                                   const LaneStack& ls, int& sp, int* err,
+                                                   const uint8_t* nbits = nullptr) {
     NodeRec n = load_node(nodes, ni);
     const float w = tk.worst_d2();
     const float m0 = mindist(n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, qx, qy, qz);
@@ -361,33 +396,32 @@ __device__ __forceinline__ void lane_push_children(const RmNode* nodes, int ni, 
         n0 = n0 & ((nb & 1u) != 0);
         n1 = n1 & ((nb & 2u) != 0);
     }
-    if (sp + 2 > maxsp) {
+    if (sp + 2 > ls.maxsp) {
         if (n0 || n1) *err = 1;
         return;
     }
     const int e0 = child_entry(ni, 0, n.r.x, n.r.z), e1 = child_entry(ni, 1, n.r.y, n.r.w);
     if (n0 && n1) {
         const bool near0 = m0 <= m1;
-        lstk[(sp++) * stride] = near0 ? e1 : e0;
-        lstk[(sp++) * stride] = near0 ? e0 : e1;
+        lane_put<OVF>(ls, sp++, near0 ? e1 : e0);
+        lane_put<OVF>(ls, sp++, near0 ? e0 : e1);
     } else if (n0) {
-        lstk[(sp++) * stride] = e0;
+        lane_put<OVF>(ls, sp++, e0);
     } else if (n1) {
-        lstk[(sp++) * stride] = e1;
+        lane_put<OVF>(ls, sp++, e1);
     }
 }
 
-template <bool NODUP = false, bool CULL = false, class Set>
-__device__ void knn_lane(const DevTree& t, float qx, float qy, float qz, Set& tk, int* lstk,
-                         int stride, int maxsp, int* err, const uint32_t* mask = nullptr) {
+template <bool NODUP = false, bool CULL = false, bool OVF = false, class Set>
+__device__ void knn_lane_s(const DevTree& t, float qx, float qy, float qz, Set& tk,
+                           const LaneStack& ls, int* err, const uint32_t* mask = nullptr) {
     int sp = 0;
     const uint8_t* nbits = CULL ? (const uint8_t*)(mask + cull_mask_words(t.n)) : nullptr;
-    lane_push_children<CULL>(t.nodes, 0, qx, qy, qz, tk, lstk, stride, sp, maxsp, err, nbits);
+    lane_push_children<CULL, OVF>(t.nodes, 0, qx, qy, qz, tk, ls, sp, err, nbits);
     while (sp > 0) {
-        const int e = lstk[(--sp) * stride];
+        const int e = lane_get<OVF>(ls, --sp);
         if (!(e & 1)) {
-            lane_push_children<CULL>(t.nodes, e >> 1, qx, qy, qz, tk, lstk, stride, sp, maxsp,
-                                     err, nbits);
+            lane_push_children<CULL, OVF>(t.nodes, e >> 1, qx, qy, qz, tk, ls, sp, err, nbits);
             continue;
         }
         const int ni = e >> 2, s = (e >> 1) & 1;
@@ -416,6 +450,15 @@ __device__ void knn_lane(const DevTree& t, float qx, float qy, float qz, Set& tk
                     make_key(dist2(v.x, v.y, v.z, qx, qy, qz), __float_as_uint(v.w)));
         }
     }
+}
+
+// Whole stack in LDS: column `lane` of a [maxsp][stride] int array.
+template <bool NODUP = false, bool CULL = false, class Set>
+__device__ __forceinline__ void knn_lane(const DevTree& t, float qx, float qy, float qz, Set& tk,
+                                         int* lstk, int stride, int maxsp, int* err,
+                                         const uint32_t* mask = nullptr) {
+    const LaneStack ls{lstk, stride, maxsp, maxsp, nullptr, lstk, 0u, 0u};
+    knn_lane_s<NODUP, CULL, false>(t, qx, qy, qz, tk, ls, err, mask);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -809,7 +852,7 @@ __device__ __forceinline__ T* opaque_ptr(T* p) {
     return p;
 }
 
-template <int MODE, bool WARM, int WPB, bool CULL>
+template <int MODE, bool WARM, int WPB, bool CULL, bool OVF = false>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) void batch_kernel(
     BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
@@ -916,12 +959,14 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                 knn_packet_rounds<kMarchK, WARM, CULL>(p.cloud.tree, px, py, pz, has, tk, wstk[w],
                                                        lq[w], p.err, (CUint*)vmask);
             } else if (has) {
+                // the first lane_lds entries of the lane's stack in LDS, deeper ones in its
+                // column of the overflow area (never reached when lane_lds == maxsp)
+                const LaneStack ls{&lstk[threadIdx.x], WPB * 64, maxsp, p.lane_lds, p.lane_ovf,
+                                   lstk, blockIdx.x * (WPB * 64), gridDim.x * (WPB * 64)};
                 if (warm_lane)
-                    knn_lane<true, CULL>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
-                                         WPB * 64, maxsp, p.err, vmask);
+                    knn_lane_s<true, CULL, OVF>(p.cloud.tree, px, py, pz, tk, ls, p.err, vmask);
                 else
-                    knn_lane<false, CULL>(p.cloud.tree, px, py, pz, tk, &lstk[threadIdx.x],
-                                          WPB * 64, maxsp, p.err, vmask);
+                    knn_lane_s<false, CULL, OVF>(p.cloud.tree, px, py, pz, tk, ls, p.err, vmask);
             }
             reread_fence();
             if (has) {
@@ -1465,7 +1510,7 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
                            q, m, k, idx, d2, maxsp, err);                                  \
         break;
 
-static int lane_stack_depth(const DevTree& t) { return t.depth + 3; }
+static int lane_stack_depth(const DevTree& t) { return lane_stack_need(t.depth); }
 
 hipError_t launch_knn(hipStream_t s, const DevTree& t, const float* q, int64_t m, int k,
                       int32_t* idx, float* d2, int* err) {
@@ -1609,12 +1654,24 @@ hipError_t launch_cull_masks(hipStream_t s, const DevTree& t, const DevView* vie
 }
 
 template <int MODE, bool WARM, int WPB>
-static hipError_t launch_batch_t(hipStream_t s, const BatchParams& p, int blocks) {
-    int maxsp = lane_stack_depth(p.cloud.tree);
-    size_t shmem = MODE == 1 ? (size_t)maxsp * WPB * 64 * sizeof(int) : 0;
-    if (p.cloud.cull)
+static hipError_t launch_batch_t(hipStream_t s, const BatchParams& p0, int blocks) {
+    BatchParams p = p0;
+    const int maxsp = lane_stack_depth(p.cloud.tree);
+    if (p.lane_lds <= 0 || p.lane_lds > maxsp) p.lane_lds = maxsp;
+    // a stack split between LDS and the overflow area needs the area
+    if (MODE == 1 && p.lane_lds < maxsp && !p.lane_ovf) return hipErrorInvalidValue;
+    size_t shmem = MODE == 1 ? (size_t)p.lane_lds * WPB * 64 * sizeof(int) : 0;
+    // the split stack has its own instantiations, so a whole-LDS stack pays nothing for it
+    const bool ovf = MODE == 1 && p.lane_lds < maxsp;
+    if (p.cloud.cull && ovf)
+        hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB, true, MODE == 1>), dim3(blocks),
+                           dim3(WPB * 64), shmem, s, p, maxsp);
+    else if (p.cloud.cull)
         hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB, true>), dim3(blocks), dim3(WPB * 64),
                            shmem, s, p, maxsp);
+    else if (ovf)
+        hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB, false, MODE == 1>), dim3(blocks),
+                           dim3(WPB * 64), shmem, s, p, maxsp);
     else
         hipLaunchKernelGGL((batch_kernel<MODE, WARM, WPB, false>), dim3(blocks), dim3(WPB * 64),
                            shmem, s, p, maxsp);

@@ -30,10 +30,36 @@ def test_library_exports_every_declared_symbol():
     assert set(_declared()) <= exported
 
 
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
+    """Size and every field offset of the ctypes mirrors equal what a C compiler makes of
+    include/rm.h, and the header's limits equal the binding's constants."""
     import ctypes as C
-    assert C.sizeof(_abi.rm_view) == 16 * 4 + 4 + 4 * 3 + 8
-    assert C.sizeof(_abi.rm_tune) == 7 * 4
+    structs = {"rm_view": _abi.rm_view, "rm_tune": _abi.rm_tune}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rm.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} sizeof %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    consts = ["RM_MAX_K", "RM_MARCH_K", "RM_NOALLOC_MAX_VIEWS", "RM_MAX_STACK_DEPTH"]
+    for c in consts:
+        lines.append(f'printf("const {c} %d\\n", (int)({c}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                             check=True).stdout.splitlines():
+        a, b, v = ln.split()
+        got[(a, b)] = int(v)
+    for name, cls in structs.items():
+        assert got[(name, "sizeof")] == C.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert got[(name, f)] == getattr(cls, f).offset, (name, f)
+    for c in consts:
+        assert got[("const", c)] == getattr(_abi, c), c
 
 
 def test_ctx_create_without_device_fails_cleanly():

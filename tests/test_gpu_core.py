@@ -474,6 +474,14 @@ TUNES = [
     dict(tile_w=16, tile_h=8, mode=1, warm=1, waves_per_block=2, blocks_per_cu=0, refill=4),
     dict(tile_w=64, tile_h=64, mode=1, warm=0, waves_per_block=4, blocks_per_cu=16, refill=64),
     dict(tile_w=24, tile_h=8, mode=0, warm=1, waves_per_block=2, blocks_per_cu=0, refill=8),
+    # per-lane stacks split between LDS and the global overflow area (1: every entry but the
+    # first overflows)
+    dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=2, blocks_per_cu=0, refill=1,
+         stack_depth=1),
+    dict(tile_w=16, tile_h=4, mode=1, warm=0, waves_per_block=4, blocks_per_cu=0, refill=1,
+         stack_depth=8),
+    dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=1, blocks_per_cu=2, refill=8,
+         stack_depth=16),
 ]
 
 
@@ -506,6 +514,23 @@ def test_batch_equals_plain(rm_ctx, room_small, ti):
             assert torch.equal(b.hit[off:off + n], phit)
             assert torch.equal(b.rgba[off:off + n], prgba)
             off += n
+    b.close()
+
+
+def test_batch_stack_depth_bounds(rm_ctx, room_small):
+    """stack_depth outside 0..RM_MAX_STACK_DEPTH is refused; values at or above the tree's
+    need keep the whole stack in LDS, and the tune reads back as given."""
+    c, views, t, m, rc, plain = room_small
+    for bad in (-1, _abi.RM_MAX_STACK_DEPTH + 1):
+        tune = dict(TUNES[3], stack_depth=bad)
+        with pytest.raises(_abi.RMError) as e:
+            api.Batch(m, views, tune=tune)
+        assert e.value.code == _abi.RM_ERR_INVALID
+    b = api.Batch(m, views, tune=dict(TUNES[3], stack_depth=_abi.RM_MAX_STACK_DEPTH))
+    assert b.tune()["stack_depth"] == _abi.RM_MAX_STACK_DEPTH
+    b.launch()
+    rm_ctx.sync()
+    assert torch.equal(b.hit, torch.cat([p[1] for p in plain]))
     b.close()
 
 
@@ -564,7 +589,7 @@ def test_culled_plain_vs_oracle(rm_ctx, room_culled):
     assert changed > 0          # culling changes pixels on these views
 
 
-@pytest.mark.parametrize("ti", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("ti", [0, 1, 2, 3, 4, 10, 11])
 def test_culled_batch_equals_plain(rm_ctx, room_culled, ti):
     c, views, t, m, rc, plain = room_culled
     b = api.Batch(m, views, tune=TUNES[ti])

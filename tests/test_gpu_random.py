@@ -20,6 +20,10 @@ TUNES = [
     dict(tile_w=32, tile_h=2, mode=1, warm=0, waves_per_block=2, blocks_per_cu=0, refill=1),
     dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=1, blocks_per_cu=1, refill=1),
     dict(tile_w=16, tile_h=16, mode=0, warm=1, waves_per_block=2, blocks_per_cu=0, refill=8),
+    dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=2, blocks_per_cu=0, refill=1,
+         stack_depth=2),
+    dict(tile_w=16, tile_h=4, mode=1, warm=0, waves_per_block=1, blocks_per_cu=0, refill=1,
+         stack_depth=5),
 ]
 
 
