@@ -218,11 +218,22 @@ __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, flo
         }
         return;
     }
+    // culling: a leaf of at most 32 points takes its mask bits from one two-word window (the
+    // mask array ends with a spare pair of words, so the second word is always there)
+    uint32_t leaf_keep = 0xffffffffu;
+    const bool one_window = CULL && cnt <= 32;
+    if (one_window) {
+        const int64_t w0 = (int64_t)b >> 5;
+        const uint64_t win = (uint64_t)mask[w0] | ((uint64_t)mask[w0 + 1] << 32);
+        leaf_keep = (uint32_t)(win >> (b & 31));
+    }
     int j = 0;
     for (; j + 4 <= cnt; j += 4) {
         // culled points (wave-uniform bits) are skipped by the scalar unit
         uint32_t keep = 0xfu;
-        if (CULL) {
+        if (CULL && one_window) {
+            keep = (leaf_keep >> j) & 0xfu;
+        } else if (CULL) {
             const int64_t i0 = (int64_t)b + j;
             const uint64_t win = (uint64_t)mask[i0 >> 5] | ((uint64_t)mask[(i0 >> 5) + 1] << 32);
             keep = (uint32_t)(win >> (i0 & 31)) & 0xfu;
