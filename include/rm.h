@@ -88,7 +88,9 @@ int rm_radius_search(rm_ctx* ctx, const rm_tree* tree, const float* q_xyzw, int6
  * cloud, self included), i.e. the distance to the nearest other point (0 for duplicates).
  * r_clamped_i = (max_radius > 0 && r_i > max_radius) ? max_radius : r_i. Both outputs are in
  * the original point order; either may be NULL. Replaces main.cu:99-149 (k = 2 search, the
- * host sqrt loop :110-112, and the clamp :144-149) as one fused kernel. */
+ * host sqrt loop :110-112, and the clamp :144-149) with two kernels: the search writes the
+ * square roots in the tree's leaf order, then a reorder writes both outputs in point order
+ * (coalesced). */
 int rm_radii(rm_ctx* ctx, const rm_tree* tree, float max_radius, float* r_unclamped,
              float* r_clamped);
 

@@ -91,7 +91,7 @@ const char* kHelp =
     "  -b,--batch INT              Views marched per batched GPU launch (default 1: one launch per view;\n"
     "                              0: the plain per-view kernel).\n"
     "Without -d (no window in this build: the viewer renders a scripted camera path):\n"
-    "  --script TEXT               Input events, one per line: '<frame> key W|S|A|D|Q|E',\n"
+    "  --script TEXT               File of input events, one per line: '<frame> key W|S|A|D|Q|E',\n"
     "                              '<frame> mouse <dx> <dy>', '<frame> scroll <dy>' (default: a built-in tour).\n"
     "  --frames INT                Frames to render (default: to the script's last event).\n"
     "  --dt FLOAT                  Fixed frame time in seconds (default: the wall clock).\n"

@@ -14,6 +14,16 @@
 //   sdf      radius gathers of the signed distance 4 B each
 // The search is exact whatever the tree, so the hit indices and depths produced here equal the
 // oracle's: the tests check that equality, which pins this restatement.
+//
+// wm_render_packet restates instead what a WAVEFRONT of the batched kernel executes in its
+// default tune (64-ray packets over 8x8 tiles, `knn_packet_rounds` with the warm start): the
+// walk enters a child if any active lane's ball reaches its box, the near child first by lane
+// majority, expands inner nodes until two leaves are queued, then re-tests and scans them; a
+// candidate is inserted by the wavefront when some lane's distance passes its k-th distance.
+// It counts per wavefront step (expansions, re-tests, scans, candidates, wavefront inserts,
+// active lanes) and per lane (the lane's own passes), and can evaluate two variants that were
+// studied and not built: the depth of the deepest node whose box holds every lane's ball
+// (`containing`), and seeding the list with the warm bound alone (`bound_only`).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -64,14 +74,16 @@ struct TopK {
     uint64_t key[32];
     void init(int kk, uint64_t pad) { k = kk; for (int j = 0; j < k; ++j) key[j] = pad; }
     float worst() const { return key_d2(key[k - 1]); }
-    void insert(uint64_t kk, bool nodup) {
-        if (!(kk < key[k - 1])) return;
+    // true when the list changed
+    bool insert(uint64_t kk, bool nodup) {
+        if (!(kk < key[k - 1])) return false;
         if (nodup)
             for (int j = 0; j < k; ++j)
-                if (uint32_t(key[j]) == uint32_t(kk)) return;
+                if (uint32_t(key[j]) == uint32_t(kk)) return false;
         int pos = k - 1;
         while (pos > 0 && kk < key[pos - 1]) { key[pos] = key[pos - 1]; --pos; }
         key[pos] = kk;
+        return true;
     }
 };
 
@@ -300,6 +312,206 @@ void wm_render(const void* t, const float* pose, float pane, int W, int H, const
         total.add(c);
     }
     std::memcpy(out, &total, sizeof(Counts));
+}
+
+typedef struct wm_packet_counts {
+    uint64_t wave_steps, lane_steps, expansions, retests, scans, candidates, wave_inserts,
+        lane_inserts, lane_child_passes, lane_candidates, rays, hits, tiles, containing_depth,
+        containing_steps;
+} wm_packet_counts;
+
+// Tiles tile_stride * j of one view (8x8, row-major); hit of each counted pixel (others keep
+// their value). bound_only: seed each warm lane's list with pads carrying the largest of its ten
+// re-measured distances instead of the ten keys (the result is the same exact top-10).
+void wm_render_packet(const void* t, const float* pose, float pane, int W, int H,
+                      int tile_stride, int bound_only, int32_t* hit, wm_packet_counts* out,
+                      int nthreads) {
+    const Tree* T = static_cast<const Tree*>(t);
+    const int tx = (W + 7) / 8, ty = (H + 7) / 8, ntiles = tx * ty;
+    const int kLeafQueue = 2;
+    const bool warm_ok = T->t.n >= kMarchK;
+    wm_packet_counts total{};
+#pragma omp parallel num_threads(nthreads_or_default(nthreads))
+    {
+        wm_packet_counts c{};
+#pragma omp for schedule(dynamic, 1)
+        for (int ti = 0; ti < ntiles; ti += std::max(1, tile_stride)) {
+            c.tiles++;
+            const int x0 = (ti % tx) * 8, y0 = (ti / tx) * 8;
+            Ray ray[64];
+            float tt[64], tfar[64], px[64], py[64], pz[64];
+            int steps[64], pix[64];
+            bool has[64], nodup[64];
+            TopK tk[64];
+            for (int l = 0; l < 64; ++l) {
+                const int x = x0 + l % 8, y = y0 + l / 8;
+                has[l] = x < W && y < H;
+                if (!has[l]) continue;
+                ray[l] = make_ray(pose, pane, W, H, x, y);
+                tt[l] = ray[l].tnear;
+                tfar[l] = std::fmin(ray[l].tfar, ray_exit_bound(*T, ray[l]));
+                steps[l] = 0;
+                pix[l] = y * W + x;
+                c.rays++;
+            }
+            while (true) {
+                int na = 0;
+                for (int l = 0; l < 64; ++l) na += has[l];
+                if (na == 0) break;
+                c.wave_steps++;
+                c.lane_steps += (uint64_t)na;
+                bool all_warm = true;
+                for (int l = 0; l < 64; ++l) {
+                    if (!has[l]) continue;
+                    px[l] = ray[l].ox + ray[l].dx * tt[l];
+                    py[l] = ray[l].oy + ray[l].dy * tt[l];
+                    pz[l] = ray[l].oz + ray[l].dz * tt[l];
+                    nodup[l] = warm_ok && steps[l] > 0;
+                    if (nodup[l]) {
+                        for (int k = 0; k < kMarchK; ++k) {
+                            const uint32_t id = uint32_t(tk[l].key[k]);
+                            const float* q = T->xyzr.data() + 4 * (size_t)id;
+                            tk[l].key[k] = make_key(dist2(q[0], q[1], q[2], px[l], py[l], pz[l]), id);
+                        }
+                        std::sort(tk[l].key, tk[l].key + kMarchK);
+                        if (bound_only) {
+                            tk[l].init(kMarchK, tk[l].key[kMarchK - 1] | 0xffffffffull);
+                            nodup[l] = false;
+                        }
+                    } else {
+                        tk[l].init(kMarchK, kPadKey);
+                        all_warm = false;
+                    }
+                }
+                if (all_warm) {
+                    // deepest node whose box holds every lane's ball with a float margin
+                    int depth = 0, ni = 0;
+                    while (true) {
+                        const RmNode& n = T->t.nodes[ni];
+                        int go = -1;
+                        for (int s = 0; s < 2 && go < 0; ++s) {
+                            if (n.cnt[s] < 0) continue;
+                            bool in = true;
+                            for (int l = 0; l < 64 && in; ++l) {
+                                if (!has[l]) continue;
+                                const float r = std::sqrt(tk[l].worst()) * 1.0001f + 1e-6f;
+                                const float* b = n.box[s];
+                                in = px[l] - b[0] > r && b[3] - px[l] > r && py[l] - b[1] > r &&
+                                     b[4] - py[l] > r && pz[l] - b[2] > r && b[5] - pz[l] > r;
+                            }
+                            if (in) go = s;
+                        }
+                        if (go < 0) break;
+                        ++depth;
+                        if (n.cnt[go] > 0) break;
+                        ni = n.ref[go];
+                    }
+                    c.containing_depth += (uint64_t)depth;
+                    c.containing_steps++;
+                }
+                int stk[512];
+                int sp = 0;
+                auto push_children = [&](int ni) {
+                    const RmNode& n = T->t.nodes[ni];
+                    c.expansions++;
+                    bool b0 = false, b1 = false;
+                    int pref0 = 0;
+                    for (int l = 0; l < 64; ++l) {
+                        if (!has[l]) continue;
+                        const float w = tk[l].worst();
+                        const float m0 = mindist(n.box[0], px[l], py[l], pz[l]);
+                        const float m1 = mindist(n.box[1], px[l], py[l], pz[l]);
+                        if (n.cnt[0] >= 0 && m0 <= w) { b0 = true; c.lane_child_passes++; }
+                        if (n.cnt[1] >= 0 && m1 <= w) { b1 = true; c.lane_child_passes++; }
+                        pref0 += m0 <= m1;
+                    }
+                    auto entry = [&](int s) {
+                        return n.cnt[s] > 0 ? ((ni << 2) | (s << 1) | 1) : (n.ref[s] << 1);
+                    };
+                    if (b0 && b1) {
+                        const bool near0 = 2 * pref0 >= na;
+                        stk[sp++] = near0 ? entry(1) : entry(0);
+                        stk[sp++] = near0 ? entry(0) : entry(1);
+                    } else if (b0) {
+                        stk[sp++] = entry(0);
+                    } else if (b1) {
+                        stk[sp++] = entry(1);
+                    }
+                };
+                push_children(0);
+                while (sp > 0) {
+                    int lq[2], nq = 0;
+                    while (sp > 0 && nq < kLeafQueue) {
+                        const int e = stk[--sp];
+                        if (e & 1)
+                            lq[nq++] = e;
+                        else
+                            push_children(e >> 1);
+                    }
+                    for (int i = 0; i < nq; ++i) {
+                        const int ni = lq[i] >> 2, s = (lq[i] >> 1) & 1;
+                        const RmNode& n = T->t.nodes[ni];
+                        c.retests++;
+                        bool any = false;
+                        for (int l = 0; l < 64; ++l)
+                            if (has[l] && mindist(n.box[s], px[l], py[l], pz[l]) <= tk[l].worst()) {
+                                any = true;
+                                c.lane_candidates += (uint64_t)n.cnt[s];
+                            }
+                        if (!any) continue;
+                        c.scans++;
+                        const float* p = T->t.pts.data() + 4 * (int64_t)n.ref[s];
+                        for (int j = 0; j < n.cnt[s]; ++j) {
+                            c.candidates++;
+                            float d[64];
+                            bool pass = false;
+                            for (int l = 0; l < 64; ++l) {
+                                if (!has[l]) continue;
+                                d[l] = dist2(p[4 * j], p[4 * j + 1], p[4 * j + 2], px[l], py[l], pz[l]);
+                                pass |= d[l] <= tk[l].worst();
+                            }
+                            if (!pass) continue;
+                            c.wave_inserts++;
+                            for (int l = 0; l < 64; ++l) {
+                                if (!has[l]) continue;
+                                c.lane_inserts += tk[l].insert(make_key(d[l], fbits(p[4 * j + 3])),
+                                                               nodup[l]);
+                            }
+                        }
+                    }
+                }
+                for (int l = 0; l < 64; ++l) {
+                    if (!has[l]) continue;
+                    float best = kInf;
+                    uint32_t bi = 0;
+                    for (int k = 0; k < kMarchK; ++k) {
+                        const uint32_t id = uint32_t(tk[l].key[k]);
+                        if (id < (uint32_t)T->t.n) {
+                            const float sd = std::sqrt(key_d2(tk[l].key[k])) - T->xyzr[4 * (size_t)id + 3];
+                            if (sd < best) { best = sd; bi = id; }
+                        }
+                    }
+                    tt[l] = tt[l] + best;
+                    ++steps[l];
+                    if (best < kMinDist) {
+                        hit[pix[l]] = (int)bi;
+                        has[l] = false;
+                        c.hits++;
+                    } else if (tt[l] > tfar[l] || steps[l] >= kMaxSteps) {
+                        hit[pix[l]] = -1;
+                        has[l] = false;
+                    }
+                }
+            }
+        }
+#pragma omp critical
+        {
+            uint64_t* a = reinterpret_cast<uint64_t*>(&total);
+            const uint64_t* b = reinterpret_cast<const uint64_t*>(&c);
+            for (size_t i = 0; i < sizeof(wm_packet_counts) / 8; ++i) a[i] += b[i];
+        }
+    }
+    *out = total;
 }
 
 }  // extern "C"

@@ -30,6 +30,17 @@ class Counts(C.Structure):
         return sum(BYTES[f] * int(getattr(self, f)) for f in BYTES)
 
 
+class PacketCounts(C.Structure):
+    """Per-wavefront-step work of the batched kernel's 64-ray packets (wm_render_packet)."""
+    _fields_ = [(f, C.c_uint64) for f in
+                ("wave_steps", "lane_steps", "expansions", "retests", "scans", "candidates",
+                 "wave_inserts", "lane_inserts", "lane_child_passes", "lane_candidates", "rays",
+                 "hits", "tiles", "containing_depth", "containing_steps")]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -53,6 +64,8 @@ def lib():
     L.wm_radii.argtypes = [vp, vp, C.POINTER(Counts), C.c_int]
     L.wm_render.argtypes = [vp, vp, C.c_float, C.c_int, C.c_int, vp, C.c_int64, vp, vp,
                             C.POINTER(Counts), C.c_int]
+    L.wm_render_packet.argtypes = [vp, vp, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, vp,
+                                   C.POINTER(PacketCounts), C.c_int]
     _lib = L
     return L
 
@@ -92,6 +105,16 @@ class WorkTree:
         lib().wm_render(self.h, _p(pose), float(pane), W, H, _p(pix) if pix is not None else None,
                         m, _p(hit), _p(depth), C.byref(c), nthreads)
         return hit, depth, c
+
+    def render_packet(self, pose_colmajor, pane, W, H, tile_stride=1, bound_only=False,
+                      nthreads=0):
+        """Hit of every pixel of the counted tiles (-2 elsewhere) and the packet counts."""
+        pose = np.ascontiguousarray(pose_colmajor, np.float32).ravel()
+        hit = np.full(W * H, -2, np.int32)
+        c = PacketCounts()
+        lib().wm_render_packet(self.h, _p(pose), float(pane), W, H, int(tile_stride),
+                               int(bool(bound_only)), _p(hit), C.byref(c), nthreads)
+        return hit, c
 
     def __del__(self):
         try:

@@ -52,3 +52,49 @@ def test_counts_on_a_single_leaf_and_on_two_leaves():
     assert (c2.nodes, c2.retests, c2.points) == (32, 64, 512)
     ru, _, _ = ob.RefTree(q, 64).radii(0.0)
     assert np.array_equal(r2, ru)
+
+
+def test_packet_walk_equals_the_oracle_and_its_counts_add_up():
+    """The packet restatement (what a wavefront of the batched kernel executes) marches the same
+    rays to the same hits as the oracle, with either seeding; a lane takes as many steps as in
+    the per-lane model, and on a one-leaf cloud every wavefront step expands the root once and
+    scans that leaf once."""
+    cloud = scenes.box_scene(20_000, seed=3)
+    p = cloud.xyzw()
+    rt = ob.RefTree(p, 64)
+    _, rc, _ = rt.radii(0.1)
+    wt = wm.WorkTree(p, 16)
+    wt.set_radii(rc)
+    v = scenes.box_scene_view(cloud, 80, 60, 60.0, 3.0, seed=2)
+    pane = ob.pane_distance(80, v.fov)
+    _, hit, _ = rt.render(rc, cloud.colors4(), v.pose_colmajor(), v.fov, 80, 60)
+    _, _, lane = wt.render(v.pose_colmajor(), pane, 80, 60)
+    for bound_only in (False, True):
+        h, c = wt.render_packet(v.pose_colmajor(), pane, 80, 60, 1, bound_only)
+        assert (h == hit).all()
+        assert c.rays == 80 * 60 and c.tiles == 10 * 8 and c.lane_steps == lane.steps
+        assert c.hits == int((hit >= 0).sum())
+        assert c.wave_steps <= c.lane_steps <= 64 * c.wave_steps
+        assert c.scans <= c.retests and c.wave_inserts <= c.candidates
+        assert c.expansions >= c.wave_steps and c.containing_steps <= c.wave_steps
+        # a seeded list only gains points it does not hold; one seeded with bounds holds none
+        assert c.lane_inserts >= (10 * c.lane_steps if bound_only else 0)
+    # every 3rd tile only (tiles 0, 3, ..., 78 of 10 x 8; the last tile row is 4 pixels tall):
+    # exactly their pixels are marched, the others keep -2
+    h, c = wt.render_packet(v.pose_colmajor(), pane, 80, 60, 3)
+    marched = np.zeros((60, 80), bool)
+    for t in range(0, 80, 3):
+        marched[(t // 10) * 8:(t // 10) * 8 + 8, (t % 10) * 8:(t % 10) * 8 + 8] = True
+    assert c.tiles == 27 and np.array_equal(h != -2, marched.ravel())
+    assert (h[marched.ravel()] == hit[marched.ravel()]).all()
+    # one leaf of 12 points (slot 0 of the root record)
+    rng = np.random.default_rng(1)
+    q = np.ones((12, 4), np.float32)
+    q[:, :3] = rng.uniform(-0.5, 0.5, (12, 3))
+    w1 = wm.WorkTree(q, 16)
+    w1.set_radii(np.full(12, 0.2, np.float32))
+    eye = scenes.look_at((0.0, 0.0, 4.0), (0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0))
+    h, c = w1.render_packet(eye.T.ravel(), ob.pane_distance(16, 0.5), 16, 8, 1)
+    assert c.tiles == 2 and c.rays == 128
+    assert c.expansions == c.wave_steps and c.retests == c.wave_steps
+    assert c.scans == c.wave_steps and c.candidates == 12 * c.wave_steps
