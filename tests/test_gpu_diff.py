@@ -32,8 +32,9 @@ def small_scene(rm_ctx):
     return cloud, rc, views, R, xyz, r, rgb
 
 
-def _disc_ok(view, xyz64, r64, hit, lin):
-    """fp64 discriminant / r^2 of each hit pixel (for selecting well-conditioned pixels)."""
+def _ray_sphere64(view, xyz64, r64, hit, lin):
+    """fp64 ray/sphere geometry of each pixel's ray and its hit sphere: (d.m, h, r) with
+    m = o - c and h = m - (d.m) d the centre-to-ray offset."""
     W, H = view.width, view.height
     pane = ob.pane_distance(W, view.fov)
     M = view.pose.astype(np.float64)
@@ -44,7 +45,19 @@ def _disc_ok(view, xyz64, r64, hit, lin):
     m = o - xyz64[hit]
     b = (d * m).sum(1)
     hv = m - b[:, None] * d
-    return (r64[hit] ** 2 - (hv ** 2).sum(1)) / r64[hit] ** 2
+    return b, hv, r64[hit]
+
+
+def _disc_ok(view, xyz64, r64, hit, lin):
+    """fp64 discriminant / r^2 of each hit pixel (for selecting well-conditioned pixels)."""
+    _, hv, rh = _ray_sphere64(view, xyz64, r64, hit, lin)
+    return (rh ** 2 - (hv ** 2).sum(1)) / rh ** 2
+
+
+def _entry_depth64(view, xyz64, r64, hit, lin):
+    """fp64 entry distance -(d.m) - sqrt(r^2 - |h|^2) of each pixel's ray into its hit sphere."""
+    b, hv, rh = _ray_sphere64(view, xyz64, r64, hit, lin)
+    return -b - np.sqrt(rh ** 2 - (hv ** 2).sum(1))
 
 
 def test_forward_matches_oracle(rm_ctx, small_scene):
@@ -197,7 +210,22 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
     coordinate and the radius. Each pixel's depth depends only on its hit point, so one
     perturbed render of the whole cloud gives every point's own finite difference: its loss
     is summed over the pixels it hits, and a point is used only where all of its pixels keep
-    their hit in both perturbed renders and sit well inside their sphere's disc."""
+    their hit in both perturbed renders and sit well inside their sphere's disc.
+
+    Every one of the 1000 points must pass three checks, with the perturbation actually
+    applied (x + h and x - h rounded to fp32) as the divisor:
+      * forward: in every perturbed render, each of the point's pixels has the GPU depth (fp32)
+        of the fp64 closed form at the same fixed hit and perturbed coordinates within 2e-5 m;
+      * adjoint: the GPU gradient equals the fp64 closed form's central difference with step
+        1e-7 m (truncation and rounding far below the tolerance) within 2e-3 of
+        max(1, |gradient|);
+      * step 1e-4: the GPU gradient equals the GPU forward's difference within 2 % plus the
+        fp32 rounding of the GPU depths (their deviations from the closed form, summed over the
+        point's pixels in both renders) plus the difference's own truncation error, measured
+        as the gap between the
+        fp64 closed form's differences at 1e-4 and at 1e-7 (the step is not small against the
+        half-chords of spheres of about 1 cm, so a central difference is off by its
+        third-derivative term)."""
     diff = _diff()
     cloud, views, max_r = scenes.workload_box_1m_diff()
     rt = ob.RefTree(cloud.xyzw(), 64)
@@ -226,12 +254,26 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
     ((d - tgt_d).abs() * m).sum().backward()
     g = np.concatenate([x.grad.cpu().numpy(), rr.grad.cpu().numpy()[:, None]], 1)
     pix_pt = hv[mask]                                # hit point of every masked pixel
+    pix_lin = [np.nonzero(mask[v])[0] for v in range(V)]
+    tgt64 = tgt_d.double().cpu().numpy().reshape(V, -1)[mask]
     npix = np.bincount(pix_pt, minlength=cloud.n)
+
+    def loss64(xp64, rp64):
+        """Per-point loss of the fp64 closed form at the fixed hits, and the depths."""
+        t64 = np.concatenate([_entry_depth64(view, xp64, rp64, hv[v][pix_lin[v]], pix_lin[v])
+                              for v, view in enumerate(views)])
+        return np.bincount(pix_pt, weights=np.abs(t64 - tgt64), minlength=cloud.n), t64
+
     step = 1e-4
     ok = (npix >= 4) & ~bad
-    fd = np.zeros((cloud.n, 4))
+    fd = np.zeros((cloud.n, 4))          # GPU forward, step 1e-4
+    fd64 = np.zeros((cloud.n, 4))        # fp64 closed form, same perturbation
+    fd64_tiny = np.zeros((cloud.n, 4))   # fp64 closed form, step 1e-7
+    noise = np.zeros((cloud.n, 4))       # fp32 rounding of the GPU depths, per point
+    maxdev = np.zeros(cloud.n)           # largest |GPU depth - fp64 depth| of a point's pixels
     for a in range(4):
-        lo = []
+        lo, lo64, coord = [], [], []
+        rnd = np.zeros(cloud.n)
         for sgn in (1.0, -1.0):
             xp, rp = xyz.clone(), r.clone()
             if a < 3:
@@ -240,20 +282,49 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
                 rp += sgn * step
             _, dp, hp = R.render(xp, rp, rgb)
             hp = hp.cpu().numpy().reshape(V, -1)
-            e = ((dp.double() - tgt_d.double()).abs()).cpu().numpy().reshape(V, -1)[mask]
-            lo.append(np.bincount(pix_pt, weights=e, minlength=cloud.n))
+            dpm = dp.double().cpu().numpy().reshape(V, -1)[mask]
+            lo.append(np.bincount(pix_pt, weights=np.abs(dpm - tgt64), minlength=cloud.n))
             # a point whose pixels change hit in either render is not comparable
             changed = np.bincount(pix_pt, weights=(hp[mask] != pix_pt), minlength=cloud.n)
             ok &= changed == 0
-        fd[:, a] = (lo[0] - lo[1]) / (2 * step)
+            xp64 = xp.double().cpu().numpy()
+            rp64 = rp.double().cpu().numpy()
+            l64, t64 = loss64(xp64, rp64)
+            lo64.append(l64)
+            dev = np.abs(dpm - t64)
+            rnd += np.bincount(pix_pt, weights=dev, minlength=cloud.n)
+            np.maximum.at(maxdev, pix_pt, dev)
+            coord.append(xp64[:, a] if a < 3 else rp64)
+        dx = coord[0] - coord[1]                     # the perturbation actually applied
+        fd[:, a] = (lo[0] - lo[1]) / dx
+        fd64[:, a] = (lo64[0] - lo64[1]) / dx
+        noise[:, a] = rnd / dx
+        tiny = []
+        for sgn in (1.0, -1.0):
+            xp64, rp64 = xyz64.copy(), r64.copy()
+            if a < 3:
+                xp64[:, a] += sgn * 1e-7
+            else:
+                rp64 += sgn * 1e-7
+            tiny.append(loss64(xp64, rp64)[0])
+        fd64_tiny[:, a] = (tiny[0] - tiny[1]) / 2e-7
     R.render(xyz, r, rgb)          # restore the tree to the unperturbed centres
     cand = np.nonzero(ok)[0]
     assert cand.size >= 1000, cand.size
     pick = rng.choice(cand, 1000, replace=False)
-    err = np.abs(fd[pick] - g[pick])
-    tol = 0.02 * np.maximum(1.0, np.abs(g[pick]))
-    assert (err <= tol).mean() > 0.995, (err / tol).max()
-    assert np.median(err / np.maximum(1.0, np.abs(g[pick]))) < 2e-3
+    g, fd, fd64, fd64_tiny, noise = g[pick], fd[pick], fd64[pick], fd64_tiny[pick], noise[pick]
+    scale = np.maximum(1.0, np.abs(g))
+    trunc = np.abs(fd64 - fd64_tiny)
+    e_adj = np.abs(g - fd64_tiny)
+    e_fd = np.abs(fd - g)
+    print(f"forward: max |depth - fp64| {maxdev[pick].max():.3e} m; adjoint vs fp64 step "
+          f"1e-7: max {(e_adj / scale).max():.3e}, median {np.median(e_adj / scale):.3e}; "
+          f"step 1e-4: max err {(e_fd / scale).max():.3e}, truncation max "
+          f"{(trunc / scale).max():.3e}, rounding max {(noise / scale).max():.3e}")
+    assert maxdev[pick].max() < 2e-5, maxdev[pick].max()
+    assert (e_adj <= 2e-3 * scale).all(), (e_adj / scale).max()
+    assert (e_fd <= 0.02 * scale + noise + trunc).all(), ((e_fd - noise - trunc) / scale).max()
+    assert np.median(e_adj / scale) < 2e-4
 
 
 def test_batch_render_diff_mixed_sizes(rm_ctx, small_scene):
