@@ -45,6 +45,8 @@ int rm_ctx_destroy(rm_ctx* ctx);
 /* Queue work on a caller-owned stream instead (e.g. torch's current stream); NULL selects the
  * device's null stream. Batch graphs are captured on the context's own stream either way. */
 int rm_ctx_set_stream(rm_ctx* ctx, void* hip_stream);
+/* The stream the context queues work on now (its own stream until rm_ctx_set_stream). */
+int rm_ctx_get_stream(const rm_ctx* ctx, void** hip_stream);
 int rm_ctx_synchronize(rm_ctx* ctx);
 const char* rm_last_error(const rm_ctx* ctx);
 /* Device time (ms) of the last call of rm_knn / rm_radius_search / rm_radii / rm_render /

@@ -49,6 +49,7 @@ SIGNATURES = {
     "rm_ctx_create": (_i32, [_i32, _pp]),
     "rm_ctx_destroy": (_i32, [_vp]),
     "rm_ctx_set_stream": (_i32, [_vp, _vp]),
+    "rm_ctx_get_stream": (_i32, [_vp, C.POINTER(_vp)]),
     "rm_ctx_synchronize": (_i32, [_vp]),
     "rm_last_error": (C.c_char_p, [_vp]),
     "rm_ctx_enable_timing": (_i32, [_vp, _i32]),

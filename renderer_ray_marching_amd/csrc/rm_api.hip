@@ -319,6 +319,12 @@ int rm_ctx_set_stream(rm_ctx* ctx, void* hip_stream) {
     return RM_OK;
 }
 
+int rm_ctx_get_stream(const rm_ctx* ctx, void** hip_stream) {
+    if (!ctx || !hip_stream) return RM_ERR_INVALID;
+    *hip_stream = (void*)ctx->stream;
+    return RM_OK;
+}
+
 int rm_ctx_synchronize(rm_ctx* ctx) {
     if (!ctx) return RM_ERR_INVALID;
     if (int r = set_device(ctx)) return r;

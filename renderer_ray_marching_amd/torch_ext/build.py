@@ -21,10 +21,10 @@ def build(verbose=False):
     libdirs = cpp_extension.library_paths()
     out = os.path.join(PKG, "_rm_torch.so")
     cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-w",
-           f"-D_GLIBCXX_USE_CXX11_ABI={abi}"]
-    cmd += [f"-I{d}" for d in inc]
+           f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-D__HIP_PLATFORM_AMD__", "-DUSE_ROCM"]
+    cmd += [f"-I{d}" for d in inc] + ["-I/opt/rocm/include"]
     cmd += [os.path.join(HERE, "rm_torch.cpp"), "-o", out + ".tmp"]
-    cmd += [f"-L{d}" for d in libdirs] + ["-lc10", "-ltorch", "-ltorch_cpu"]
+    cmd += [f"-L{d}" for d in libdirs] + ["-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu"]
     cmd += [f"-L{PKG}", "-l:librm_hip.so", "-Wl,-rpath,$ORIGIN"]
     if verbose:
         print(" ".join(cmd))

@@ -3,7 +3,10 @@
 // analytic sphere depth, flat point colour) and whose backward runs the batch adjoint
 // (rm_batch_render_adjoint: one sort of every view's hit pixels by point, one reduction per
 // point) at the centres and radii the forward marched. All arithmetic runs in librm_hip.so;
-// this file only moves tensors across the C ABI.
+// this file only moves tensors across the C ABI. Both directions run on the caller's current
+// stream (the context is pointed at it for the call and back at its own stream after it), so
+// they are ordered with the torch work that produces their inputs and consumes their outputs.
+#include <c10/hip/HIPStream.h>
 #include <torch/script.h>
 #include <torch/torch.h>
 
@@ -99,6 +102,21 @@ struct DiffScene : torch::CustomClassHolder {
     }
 };
 
+// Points the library context at torch's current stream on the tensors' device for the lifetime
+// of the guard, then back at the stream it had.
+struct CurrentStream {
+    rm_ctx* ctx;
+    void* prev = nullptr;
+    CurrentStream(rm_ctx* c, const torch::Tensor& t) : ctx(c) {
+        check_rm(rm_ctx_get_stream(ctx, &prev), ctx, "rm_ctx_get_stream");
+        hipStream_t st = c10::hip::getCurrentHIPStream(t.device().index()).stream();
+        check_rm(rm_ctx_set_stream(ctx, reinterpret_cast<void*>(st)), ctx, "rm_ctx_set_stream");
+    }
+    ~CurrentStream() { (void)rm_ctx_set_stream(ctx, prev); }
+    CurrentStream(const CurrentStream&) = delete;
+    CurrentStream& operator=(const CurrentStream&) = delete;
+};
+
 void bind_params(DiffScene& s, const torch::Tensor& x, const torch::Tensor& r,
                  const torch::Tensor& rgb) {
     auto c4 = torch::cat({rgb, torch::ones({s.n, 1}, rgb.options())}, 1).contiguous();
@@ -131,6 +149,7 @@ class RenderFunction : public torch::autograd::Function<RenderFunction> {
         // moves the cloud; in-place edits of these tensors make autograd raise)
         auto x = xyz.contiguous();
         auto r = radii.contiguous().reshape({s.n});
+        CurrentStream on_stream(s.ctx, x);
         bind_params(s, x, r, rgb);
         auto fopt = xyz.options();
         const int64_t P = s.view_offsets().back();
@@ -164,6 +183,7 @@ class RenderFunction : public torch::autograd::Function<RenderFunction> {
         auto saved = actx->get_saved_variables();
         auto hit = saved[0].reshape({-1});
         auto x = saved[1], r = saved[2];
+        CurrentStream on_stream(s.ctx, x);
         const int64_t P = hit.numel();
         auto fopt = hit.options().dtype(torch::kFloat32);
         auto g_img = grads[0].defined() ? grads[0].reshape({P, 3}).contiguous()

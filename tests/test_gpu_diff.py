@@ -259,9 +259,12 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
     npix = np.bincount(pix_pt, minlength=cloud.n)
 
     def loss64(xp64, rp64):
-        """Per-point loss of the fp64 closed form at the fixed hits, and the depths."""
-        t64 = np.concatenate([_entry_depth64(view, xp64, rp64, hv[v][pix_lin[v]], pix_lin[v])
-                              for v, view in enumerate(views)])
+        """Per-point loss of the fp64 closed form at the fixed hits, and the depths. A pixel
+        near its sphere's rim (a point marked bad, never picked) can lose its intersection
+        under a perturbation: its depth is NaN and so are that point's sums."""
+        with np.errstate(invalid="ignore"):
+            t64 = np.concatenate([_entry_depth64(view, xp64, rp64, hv[v][pix_lin[v]],
+                                                 pix_lin[v]) for v, view in enumerate(views)])
         return np.bincount(pix_pt, weights=np.abs(t64 - tgt64), minlength=cloud.n), t64
 
     step = 1e-4
@@ -293,7 +296,8 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
             lo64.append(l64)
             dev = np.abs(dpm - t64)
             rnd += np.bincount(pix_pt, weights=dev, minlength=cloud.n)
-            np.maximum.at(maxdev, pix_pt, dev)
+            with np.errstate(invalid="ignore"):
+                np.maximum.at(maxdev, pix_pt, dev)
             coord.append(xp64[:, a] if a < 3 else rp64)
         dx = coord[0] - coord[1]                     # the perturbation actually applied
         fd[:, a] = (lo[0] - lo[1]) / dx
@@ -313,6 +317,7 @@ def test_central_fd_1000_points_full_workload(rm_ctx):
     assert cand.size >= 1000, cand.size
     pick = rng.choice(cand, 1000, replace=False)
     g, fd, fd64, fd64_tiny, noise = g[pick], fd[pick], fd64[pick], fd64_tiny[pick], noise[pick]
+    assert np.isfinite(fd64).all() and np.isfinite(fd64_tiny).all() and np.isfinite(noise).all()
     scale = np.maximum(1.0, np.abs(g))
     trunc = np.abs(fd64 - fd64_tiny)
     e_adj = np.abs(g - fd64_tiny)
@@ -361,6 +366,41 @@ def test_batch_render_diff_mixed_sizes(rm_ctx, small_scene):
         assert torch.equal(dep_b[off:off + n], d1.reshape(-1))
         off += n
     b.close()
+
+
+def test_op_runs_on_the_current_stream(rm_ctx, small_scene):
+    """Forward and backward called on a side stream (torch.cuda.stream) give the default
+    stream's outputs and gradients bit for bit, and the context queues on the stream it had
+    before the calls afterwards."""
+    import ctypes as C
+    from renderer_ray_marching_amd import _abi
+    cloud, rc, views, R, xyz, r, rgb = small_scene
+
+    def run():
+        x = xyz.clone().requires_grad_(True)
+        rr = r.clone().requires_grad_(True)
+        c = rgb.clone().requires_grad_(True)
+        img, d, h = R.render(x, rr, c)
+        (img.sum() + d.sum()).backward()
+        return [img.detach().clone(), d.detach().clone(), h.clone(), x.grad.clone(),
+                rr.grad.clone(), c.grad.clone()]
+
+    before = C.c_void_p()
+    assert _abi.lib().rm_ctx_get_stream(rm_ctx.h, C.byref(before)) == 0
+    ref = run()
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        got = run()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    after = C.c_void_p()
+    assert _abi.lib().rm_ctx_get_stream(rm_ctx.h, C.byref(after)) == 0
+    assert after.value == before.value
+    assert (ref[2] >= 0).any()
+    for a, b in zip(ref, got):
+        assert torch.equal(a, b)
 
 
 def test_batched_forward_equals_per_view(rm_ctx, small_scene):
