@@ -167,12 +167,21 @@ typedef struct rm_tune {
                                  (depth + 3), keeps the whole stack in LDS. Fewer LDS entries
                                  leave room for more resident wavefronts. Mode 0 ignores it.
                                  Results are identical for every value. */
+    int32_t exit_depth;       /* early miss: a ray ends as a miss once it has left, for good,
+                                 every box of the tree's 2^exit_depth slots at that depth, each
+                                 dilated by the largest radius and a margin (0..RM_MAX_EXIT_DEPTH;
+                                 0: the box of the whole cloud). Pays where many rays pass
+                                 between the cloud's parts. Results are identical for every
+                                 value. */
 } rm_tune;
 #define RM_MAX_STACK_DEPTH 64
+#define RM_MAX_EXIT_DEPTH 10
 
 /* A batch of views marched by one persistent kernel over a shared tile queue, launched as a
  * captured HIP graph. rgba / hit (hit may be NULL) are the concatenated outputs, sized by the
- * views' offsets and sizes; they are bound at creation (the graph captures them). */
+ * views' offsets and sizes; they are bound at creation (the graph captures them). tune may be
+ * NULL: the default tune, with exit_depth 6 when at least half of the cameras are outside the
+ * box of all centres and 0 otherwise. */
 int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
                     const rm_tune* tune, float* rgba, int32_t* hit, rm_batch** out);
 /* Time candidate rm_tune settings on every sample_stride-th view (whole views, so the sample

@@ -21,6 +21,7 @@ RM_MAX_K = 32
 RM_MARCH_K = 10
 RM_NOALLOC_MAX_VIEWS = 1024
 RM_MAX_STACK_DEPTH = 64
+RM_MAX_EXIT_DEPTH = 10
 
 
 class rm_view(C.Structure):
@@ -32,7 +33,7 @@ class rm_tune(C.Structure):
     _fields_ = [("tile_w", C.c_int32), ("tile_h", C.c_int32), ("mode", C.c_int32),
                 ("warm", C.c_int32), ("waves_per_block", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("refill", C.c_int32),
-                ("stack_depth", C.c_int32)]
+                ("stack_depth", C.c_int32), ("exit_depth", C.c_int32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

@@ -25,6 +25,9 @@ constexpr int kWaveStack = 64;
 #define RM_LEAF_QUEUE 2
 #endif
 constexpr int kLeafQueue = RM_LEAF_QUEUE;
+// Early-miss frontier of the batched march: entries of this many floats, depth at most this.
+constexpr int kFrontierFloats = 8;
+constexpr int kMaxFrontierDepth = 10;
 
 
 // Device view of a tree + cloud, passed by value to kernels.
@@ -93,6 +96,10 @@ struct BatchParams {
     // layout); null when the whole stack is in LDS
     int32_t lane_lds;
     int* lane_ovf;
+    // early miss: 0 = the root box; else the frontier's entry count (2^depth) and its array
+    // (a header entry, then the entries), refreshed from the tree at every launch
+    int32_t nfrontier;
+    const float* frontier;
 };
 
 // Per-lane stack entries a traversal of a tree of this depth needs (root children, then at most
@@ -141,6 +148,8 @@ hipError_t launch_render_plain(hipStream_t s, const DevCloud& c, const DevView& 
 hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm, int wpb,
                         int blocks);
 hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q);
+// frontier of depth 1..kMaxFrontierDepth: (1 + 2^depth) * kFrontierFloats floats at out
+hipError_t launch_frontier(hipStream_t s, const RmNode* nodes, int depth, float* out);
 hipError_t launch_store_views(hipStream_t s, const ViewChunk& ch, DevView* table);
 hipError_t launch_cull_masks(hipStream_t s, const DevTree& t, const DevView* views, int nviews,
                              uint32_t* cull, int64_t stride, const TreeLevels& lv);

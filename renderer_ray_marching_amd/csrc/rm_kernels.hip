@@ -686,6 +686,57 @@ __device__ __forceinline__ float ray_exit_bound(const DevCloud& c, const Ray& r)
     return tout + fabsf(tout) * 1e-5f + 1e-3f;
 }
 
+// Early miss over a frontier of the tree: the union of the tight boxes of the tree's child slots
+// at depth D (a leaf met above that depth stands for itself). Every centre lies in one of them,
+// so past the ray's exit from the last of those boxes (each dilated as in ray_exit_bound) that
+// it crosses, it is outside every dilated box for good (a ray leaves a convex box once) and no
+// sphere can be hit any more; -INF when it crosses none. The frontier is an array of
+// kFrontierFloats-float entries (lo.xyz, hi.xyz, valid, unused) after one header entry whose
+// first float counts entries with a non-finite coordinate (then the root box is used instead).
+// Rays with a direction component below 1e-20 in magnitude also use the root box, so that the
+// slab arithmetic below stays finite.
+__device__ __forceinline__ float frontier_exit(const DevCloud& c, const Ray& r, CFloat* fr,
+                                               int nfr) {
+    if (fr[0] != 0.0f) return ray_exit_bound(c, r);
+    const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+    if (!(fabsf(d[0]) >= 1e-20f && fabsf(d[1]) >= 1e-20f && fabsf(d[2]) >= 1e-20f))
+        return ray_exit_bound(c, r);
+    CFloat* f = (CFloat*)c.tree.nodes;
+    CInt* fi = (CInt*)f;
+    float ext = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+        if (fi[14 + s] >= 0)
+#pragma unroll
+            for (int a = 0; a < 6; ++a) ext = fmaxf(ext, fabsf(f[6 * s + a]));
+    const float scale = fmaxf(fmaxf(fabsf(r.ox), fabsf(r.oy)), fabsf(r.oz)) + ext + 1000.0f;
+    const float e = __uint_as_float(*(const __attribute__((address_space(4))) uint32_t*)c.rmax) +
+                    2.0f * kMinDist + 1e-5f * scale;
+    float inv[3], c1[3], c2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        inv[a] = 1.0f / d[a];
+        c1[a] = (-e - o[a]) * inv[a];
+        c2[a] = (e - o[a]) * inv[a];
+    }
+    float best = -kInf;
+    for (int i = 1; i <= nfr; ++i) {
+        CFloat* b = fr + (int64_t)i * kFrontierFloats;
+        if (b[6] == 0.0f) continue;
+        float tin = -kInf, tout = kInf;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float t1 = __builtin_fmaf(b[a], inv[a], c1[a]);
+            const float t2 = __builtin_fmaf(b[3 + a], inv[a], c2[a]);
+            tin = fmaxf(tin, fminf(t1, t2));
+            tout = fminf(tout, fmaxf(t1, t2));
+        }
+        if (tin <= tout) best = fmaxf(best, tout);
+    }
+    if (best == -kInf) return -kInf;
+    return best + fabsf(best) * 1e-5f + 1e-3f;
+}
+
 // Signed distance of src/ray_marching.cu:280-290 over the sorted kNN-10: strict minimum of
 // sqrtf(d2) - radius, first entry wins ties; initial best (+INF, 0).
 __device__ __forceinline__ void sdf10(const DevCloud& c, const TopK<kMarchK>& tk, float& s_out,
@@ -936,7 +987,11 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                         rs.dy[w][lane] = r.dy;
                         rs.dz[w][lane] = r.dz;
                         rs.t[w][lane] = r.tnear;
-                        rs.tfar[w][lane] = fminf(r.tfar, ray_exit_bound(cl, r));
+                        rs.tfar[w][lane] = fminf(
+                            r.tfar, p.nfrontier > 0
+                                        ? frontier_exit(cl, r, (CFloat*)opaque_ptr(p.frontier),
+                                                        opaque_s(p.nfrontier))
+                                        : ray_exit_bound(cl, r));
                         rs.steps[w][lane] = 0;
                         rs.pix[w][lane] = y * Wr + x;
                         has = true;
@@ -1089,6 +1144,51 @@ __global__ __launch_bounds__(256) void cull_nodes_kernel(const RmNode* __restric
 }
 
 __global__ void reset_queue_kernel(unsigned long long* q) { *q = 0ull; }
+
+// Frontier of the early miss (see frontier_exit): entry 1 + f holds the box of the child slot
+// reached from the root by the depth bits of f (most significant first); a leaf met above depth
+// `depth` is written by the first entry of its range, the others and empty slots are invalid.
+// One block of 2^depth threads (depth <= 10); thread 0 writes the header.
+__global__ __launch_bounds__(1024) void frontier_kernel(const RmNode* __restrict__ nodes,
+                                                        int depth, float* __restrict__ out) {
+    __shared__ int nonfinite;
+    const int f = threadIdx.x;
+    if (f == 0) nonfinite = 0;
+    __syncthreads();
+    float box[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool valid = false;
+    int ni = 0;
+    for (int level = 0; level < depth; ++level) {
+        const int slot = (f >> (depth - 1 - level)) & 1;
+        const int rem = depth - 1 - level;
+        const int cnt = nodes[ni].cnt[slot];
+        if (cnt < 0 || (cnt > 0 && rem > 0 && (f & ((1 << rem) - 1)) != 0)) break;
+        if (cnt > 0 || rem == 0) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) box[a] = nodes[ni].box[slot][a];
+            valid = true;
+            break;
+        }
+        ni = nodes[ni].ref[slot];
+    }
+    if (valid) {
+        bool fin = true;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) fin &= fabsf(box[a]) <= 3.0e38f;
+        if (!fin) atomicAdd(&nonfinite, 1);
+    }
+    float* o = out + (int64_t)(1 + f) * kFrontierFloats;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) o[a] = box[a];
+    o[6] = valid ? 1.0f : 0.0f;
+    o[7] = 0.0f;
+    __syncthreads();
+    if (f == 0) {
+        out[0] = (float)nonfinite;
+#pragma unroll
+        for (int a = 1; a < kFrontierFloats; ++a) out[a] = 0.0f;
+    }
+}
 
 __global__ void store_views_kernel(ViewChunk ch, DevView* __restrict__ table) {
     const int i = threadIdx.x;
@@ -1699,6 +1799,12 @@ hipError_t launch_batch(hipStream_t s, const BatchParams& p, int mode, int warm,
     RM_B(1, true, 1) RM_B(1, true, 2) RM_B(1, true, 4)
 #undef RM_B
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_frontier(hipStream_t s, const RmNode* nodes, int depth, float* out) {
+    if (depth < 1 || depth > kMaxFrontierDepth) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(frontier_kernel, dim3(1), dim3(1 << depth), 0, s, nodes, depth, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset_queue(hipStream_t s, unsigned long long* q) {

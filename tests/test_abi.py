@@ -40,7 +40,8 @@ def test_struct_layouts_match_header(tmp_path):
         lines.append(f'printf("{name} sizeof %zu\\n", sizeof({name}));')
         for f, _ in cls._fields_:
             lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
-    consts = ["RM_MAX_K", "RM_MARCH_K", "RM_NOALLOC_MAX_VIEWS", "RM_MAX_STACK_DEPTH"]
+    consts = ["RM_MAX_K", "RM_MARCH_K", "RM_NOALLOC_MAX_VIEWS", "RM_MAX_STACK_DEPTH",
+              "RM_MAX_EXIT_DEPTH"]
     for c in consts:
         lines.append(f'printf("const {c} %d\\n", (int)({c}));')
     lines.append("return 0; }")

@@ -482,6 +482,13 @@ TUNES = [
          stack_depth=8),
     dict(tile_w=8, tile_h=8, mode=1, warm=1, waves_per_block=1, blocks_per_cu=2, refill=8,
          stack_depth=16),
+    # early miss over a frontier of the tree instead of the whole cloud's box
+    dict(tile_w=8, tile_h=8, mode=0, warm=1, waves_per_block=2, blocks_per_cu=0, refill=1,
+         exit_depth=6),
+    dict(tile_w=16, tile_h=8, mode=1, warm=0, waves_per_block=4, blocks_per_cu=0, refill=4,
+         exit_depth=10),
+    dict(tile_w=8, tile_h=8, mode=0, warm=0, waves_per_block=1, blocks_per_cu=0, refill=1,
+         exit_depth=1),
 ]
 
 
@@ -532,6 +539,59 @@ def test_batch_stack_depth_bounds(rm_ctx, room_small):
     rm_ctx.sync()
     assert torch.equal(b.hit, torch.cat([p[1] for p in plain]))
     b.close()
+
+
+def test_early_miss_frontier_keeps_results(rm_ctx):
+    """The early miss over a frontier of the tree (exit_depth 1..RM_MAX_EXIT_DEPTH) gives the
+    plain march's outputs on every pixel: box scenes seen from outside (many rays pass between
+    the boxes), an axis-aligned camera (rays with a direction component of exactly 0 take the
+    cloud box's bound), a cloud smaller than one leaf, and after a refit that moves every point
+    (the frontier is read from the tree at each launch). Depths out of range are refused."""
+    def views_of(cloud, n=3):
+        vs = [scenes.box_scene_view(cloud, 96, 72, 80.0, distance=4.0, seed=s)
+              for s in range(1, n + 1)]
+        pose = np.eye(4, dtype=np.float32)
+        pose[:3, 3] = (cloud.xyz.astype(np.float64).mean(0) + [0.0, 0.0, 5.0]).astype(np.float32)
+        vs.append(scenes.View(pose, 64, 48, 40.0, name="axis_aligned"))
+        return vs
+
+    def check(m, views, depths):
+        plain = []
+        for v in views:
+            rgba, hit = m.render_to_texture(v.pose_colmajor(), v.fov, v.width, v.height)
+            plain.append((rgba.reshape(-1, 4).clone(), hit.reshape(-1).clone()))
+        rm_ctx.sync()
+        ref_hit = torch.cat([p[1] for p in plain])
+        ref_rgba = torch.cat([p[0] for p in plain])
+        for ed in depths:
+            b = api.Batch(m, views, tune=dict(TUNES[1], exit_depth=ed))
+            for rep in range(2):
+                b.hit.fill_(-7)
+                b.launch()
+                rm_ctx.sync()
+                assert torch.equal(b.hit, ref_hit), ed
+                assert torch.equal(b.rgba, ref_rgba), ed
+            b.close()
+        return ref_hit
+
+    cloud = scenes.box_scene(30_000, seed=9)
+    t, m, rc = _marcher(rm_ctx, cloud, 0.05)
+    views = views_of(cloud)
+    hit = check(m, views, (0, 1, 3, 6, _abi.RM_MAX_EXIT_DEPTH))
+    assert (hit < 0).float().mean() > 0.2          # many rays leave the cloud
+    for bad in (-1, _abi.RM_MAX_EXIT_DEPTH + 1):
+        with pytest.raises(_abi.RMError) as e:
+            api.Batch(m, views, tune=dict(TUNES[1], exit_depth=bad))
+        assert e.value.code == _abi.RM_ERR_INVALID
+    # the points moved by 0.2 m along x: a frontier kept from before the refit would cut rays
+    # that now hit the shifted boxes
+    moved = torch.from_numpy(cloud.xyz + np.float32([0.2, 0.0, 0.0])).cuda()
+    m.refit(moved)
+    check(m, views, (4, 8))
+    # a cloud smaller than one leaf: the frontier is that leaf alone
+    tiny = scenes.box_scene(7, seed=3)
+    t2, m2, rc2 = _marcher(rm_ctx, tiny, 0.5)
+    check(m2, views_of(tiny, 1), (2, 6))
 
 
 def test_batch_plain_vs_oracle_subsample(rm_ctx, room_small):
@@ -589,7 +649,7 @@ def test_culled_plain_vs_oracle(rm_ctx, room_culled):
     assert changed > 0          # culling changes pixels on these views
 
 
-@pytest.mark.parametrize("ti", [0, 1, 2, 3, 4, 10, 11])
+@pytest.mark.parametrize("ti", [0, 1, 2, 3, 4, 10, 11, 13, 14])
 def test_culled_batch_equals_plain(rm_ctx, room_culled, ti):
     c, views, t, m, rc, plain = room_culled
     b = api.Batch(m, views, tune=TUNES[ti])
