@@ -167,12 +167,13 @@ typedef struct rm_tune {
                                  (depth + 3), keeps the whole stack in LDS. Fewer LDS entries
                                  leave room for more resident wavefronts. Mode 0 ignores it.
                                  Results are identical for every value. */
-    int32_t exit_depth;       /* early miss: a ray ends as a miss once it has left, for good,
-                                 every box of the tree's 2^exit_depth slots at that depth, each
-                                 dilated by the largest radius and a margin (0..RM_MAX_EXIT_DEPTH;
-                                 0: the box of the whole cloud). Pays where many rays pass
-                                 between the cloud's parts. Results are identical for every
-                                 value. */
+    int32_t exit_depth;       /* early miss of views whose camera is outside the box of all
+                                 centres: a ray ends as a miss once it has left, for good, every
+                                 box of the tree's 2^exit_depth slots at that depth, each dilated
+                                 by the largest radius and a margin (0..RM_MAX_EXIT_DEPTH; 0: the
+                                 box of the whole cloud, which views from inside always use).
+                                 Pays where many rays pass between the cloud's parts (an object
+                                 seen from around it). Results are identical for every value. */
 } rm_tune;
 #define RM_MAX_STACK_DEPTH 64
 #define RM_MAX_EXIT_DEPTH 10
@@ -180,8 +181,7 @@ typedef struct rm_tune {
 /* A batch of views marched by one persistent kernel over a shared tile queue, launched as a
  * captured HIP graph. rgba / hit (hit may be NULL) are the concatenated outputs, sized by the
  * views' offsets and sizes; they are bound at creation (the graph captures them). tune may be
- * NULL: the default tune, with exit_depth 6 when at least half of the cameras are outside the
- * box of all centres and 0 otherwise. */
+ * NULL: the default tune. */
 int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, int nviews,
                     const rm_tune* tune, float* rgba, int32_t* hit, rm_batch** out);
 /* Time candidate rm_tune settings on every sample_stride-th view (whole views, so the sample
@@ -199,9 +199,10 @@ int rm_render_batch(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
 /* Allocation-free form of rm_render_batch. It has no counterpart in the reference; it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). Same outputs as rm_render_batch (default tune, one
- * launch of the persistent march, no graph), but the tile-queue counter and the view table live
- * in a static device array of the library, one per GPU, sized for RM_NOALLOC_MAX_VIEWS views
- * (256 B + 160 B per view, 160.25 KiB); it calls no hipMalloc, hipFree or allocating library
+ * launch of the persistent march, no graph), but the tile-queue counter, the view table and the
+ * default tune's early-miss frontier live in a static device array of the library, one per GPU,
+ * sized for RM_NOALLOC_MAX_VIEWS views (256 B + 160 B per view + 2,080 B, 162.3 KiB); it calls
+ * no hipMalloc, hipFree or allocating library
  * routine. More views than that, or a cloud with culling enabled (its culling data grow with
  * the cloud), are rejected with RM_ERR_UNSUPPORTED. Every call on a GPU shares that GPU's
  * array and overwrites it: calls ordered on one stream are safe, calls that overlap on one GPU

@@ -20,13 +20,21 @@ using rm::DevTree;
 using rm::DevView;
 
 // Scratch of rm_render_batch_noalloc: the tile-queue counter (first 256 bytes), then the view
-// table of at most RM_NOALLOC_MAX_VIEWS views. A file-scope device array: every GPU the library
+// table of at most RM_NOALLOC_MAX_VIEWS views, then the early-miss frontier of the default tune's
+// depth (refreshed from the tree at every call). A file-scope device array: every GPU the library
 // runs on holds its own copy, at its own address, until the process ends. In host code the name
 // `scratch` is only a stand-in for the symbol: kernels, copies and fills use the device address
 // that hipGetSymbolAddress returns for the current device (kept per device by scratch_lookup
 // below), never the name.
 constexpr size_t kScratchHeader = 256;
-constexpr size_t kScratchBytes = kScratchHeader + (size_t)RM_NOALLOC_MAX_VIEWS * sizeof(DevView);
+// the early-miss frontier of the default tune's depth follows the view table
+constexpr int kDefaultExitDepth = 6;
+constexpr size_t kScratchViews = (size_t)RM_NOALLOC_MAX_VIEWS * sizeof(DevView);
+constexpr size_t kScratchFrontier =
+    sizeof(float) * rm::kFrontierFloats * ((size_t)1 + ((size_t)1 << kDefaultExitDepth));
+constexpr size_t kScratchBytes = kScratchHeader + kScratchViews + kScratchFrontier;
+static_assert(kDefaultExitDepth <= rm::kMaxFrontierDepth && kScratchViews % 256 == 0,
+              "the frontier follows the view table, 256-byte aligned");
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 struct rm_ctx {
@@ -709,38 +717,8 @@ rm_tune default_tune() {
     t.blocks_per_cu = 0;
     t.refill = 1;
     t.stack_depth = 0;
-    t.exit_depth = 0;
+    t.exit_depth = kDefaultExitDepth;
     return t;
-}
-
-// Early-miss depth of a batch created without a tune. The frontier pays where rays pass between
-// the cloud's parts and leave it, as they do for cameras outside the cloud's box (an object seen
-// from around it: 12 % of the march on the 1M-point box workload at depth 6); from inside the box
-// (a room seen from within) nearly every ray meets a surface and the frontier's cost (0.8 % at
-// depth 6 on the 10M-point room) is not repaid. Depth 6 when at least half of the cameras are
-// outside the box of all centres, else 0.
-int default_exit_depth(rm_ctx* ctx, const rm_cloud* c, const rm_view* views, int nviews,
-                       int* out) {
-    RmNode root;
-    RM_HIP(ctx, hipMemcpyAsync(&root, c->tree->nodes, sizeof(RmNode), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    RM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int s = 0; s < 2; ++s)
-        if (root.cnt[s] >= 0)
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = std::min(lo[a], root.box[s][a]);
-                hi[a] = std::max(hi[a], root.box[s][3 + a]);
-            }
-    int outside = 0;
-    for (int i = 0; i < nviews; ++i) {
-        const float* o = views[i].pose + 12;
-        bool in = true;
-        for (int a = 0; a < 3; ++a) in = in && o[a] >= lo[a] && o[a] <= hi[a];
-        outside += in ? 0 : 1;
-    }
-    *out = 2 * outside >= nviews ? 6 : 0;
-    return RM_OK;
 }
 
 static_assert(RM_MAX_EXIT_DEPTH == rm::kMaxFrontierDepth,
@@ -888,8 +866,6 @@ int rm_batch_create(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* views, in
     if (!tune_valid(t)) return fail(ctx, RM_ERR_INVALID, "rm_batch_create: bad tune");
     if (t.warm && cloud->n < rm::kMarchK) t.warm = 0;
     if (int r = set_device(ctx)) return r;
-    if (!tune)
-        if (int r = default_exit_depth(ctx, cloud, views, nviews, &t.exit_depth)) return r;
     rm_batch* b = new (std::nothrow) rm_batch();
     if (!b) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");
     b->ctx = ctx;
@@ -1229,10 +1205,14 @@ int rm_render_batch_noalloc(rm_ctx* ctx, const rm_cloud* cloud, const rm_view* v
     p.out.hit = hit;
     p.lane_lds = 0;          // the default tune keeps any per-lane stack whole in LDS
     p.lane_ovf = nullptr;
-    p.nfrontier = 0;         // the root box's early miss (a frontier would need device memory)
-    p.frontier = nullptr;
+    float* d_frontier = reinterpret_cast<float*>(ctx->scratch + kScratchHeader + kScratchViews);
+    p.nfrontier = t.exit_depth > 0 ? (1 << t.exit_depth) : 0;
+    p.frontier = d_frontier;
     timing_begin(ctx);
     RM_HIP(ctx, rm::launch_reset_queue(ctx->stream, p.queue));
+    if (t.exit_depth > 0)
+        RM_HIP(ctx, rm::launch_frontier(ctx->stream, cloud->tree->nodes, t.exit_depth,
+                                        d_frontier));
     RM_HIP(ctx, rm::launch_batch(ctx->stream, p, t.mode, t.warm, t.waves_per_block,
                                  ctx->num_cus * std::max(1, 32 / t.waves_per_block)));
     timing_end(ctx);

@@ -696,6 +696,28 @@ __device__ __forceinline__ float ray_exit_bound(const DevCloud& c, const Ray& r)
 // Rays with a direction component below 1e-20 in magnitude also use the root box, so that the
 // slab arithmetic below stays finite.
 __device__ __forceinline__ float frontier_exit(const DevCloud& c, const Ray& r, CFloat* fr,
+                                               int nfr);
+
+// Whether a camera at (x, y, z) is outside the box of all centres (the root's two child boxes).
+// The batched march uses the frontier for such views only: from inside the cloud's box (a room
+// seen from within) nearly every ray meets a surface and the frontier's cost is not repaid.
+__device__ __forceinline__ bool camera_outside_cloud(const DevCloud& c, float x, float y,
+                                                     float z) {
+    CFloat* f = (CFloat*)c.tree.nodes;
+    CInt* fi = (CInt*)f;
+    float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+        if (fi[14 + s] >= 0)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], f[6 * s + a]);
+                hi[a] = fmaxf(hi[a], f[6 * s + 3 + a]);
+            }
+    return !(x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2]);
+}
+
+__device__ __forceinline__ float frontier_exit(const DevCloud& c, const Ray& r, CFloat* fr,
                                                int nfr) {
     if (fr[0] != 0.0f) return ray_exit_bound(c, r);
     const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
@@ -987,8 +1009,10 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                         rs.dy[w][lane] = r.dy;
                         rs.dz[w][lane] = r.dz;
                         rs.t[w][lane] = r.tnear;
+                        const bool use_frontier =
+                            p.nfrontier > 0 && camera_outside_cloud(cl, Vr[12], Vr[13], Vr[14]);
                         rs.tfar[w][lane] = fminf(
-                            r.tfar, p.nfrontier > 0
+                            r.tfar, use_frontier
                                         ? frontier_exit(cl, r, (CFloat*)opaque_ptr(p.frontier),
                                                         opaque_s(p.nfrontier))
                                         : ray_exit_bound(cl, r));

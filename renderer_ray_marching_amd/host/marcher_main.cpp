@@ -17,8 +17,10 @@
 //   * a render key with a single path component writes into <output> itself;
 //   * without -d there is no window to open: the viewer's compute half runs instead. Its FPS
 //     camera (src/camera.h) is driven by a scripted input (--script, or a built-in tour), every
-//     frame is rendered at 1024 x 768 (src/common.h:13-14) into a device buffer with rm_render
-//     at inverse(view) and the camera's zoom as field of view (main.cu:249-258), and "<n> FPS"
+//     frame is rendered at 1024 x 768 (src/common.h:13-14) into a device buffer at
+//     inverse(view) and the camera's zoom as field of view (main.cu:249-258) — by the batched
+//     kernel through rm_render_batch_noalloc (nothing allocated per frame), or with -b 0 by the
+//     plain kernel (rm_render), the same image either way — and "<n> FPS"
 //     is printed once a second as FPSCounter::update does (src/utils.cu:104-112); --dump
 //     writes frames and their poses for inspection.
 #include <fcntl.h>
@@ -89,7 +91,7 @@ const char* kHelp =
     "  -p,--precompute-radii       Precompute radii even if already precomputed.\n"
     "  -i,--ignore_existing        Ignore existing renders and forcefully rewrite them.\n"
     "  -b,--batch INT              Views marched per batched GPU launch (default 1: one launch per view;\n"
-    "                              0: the plain per-view kernel).\n"
+    "                              0: the plain per-view kernel, also for the viewer's frames).\n"
     "Without -d (no window in this build: the viewer renders a scripted camera path):\n"
     "  --script TEXT               File of input events, one per line: '<frame> key W|S|A|D|Q|E',\n"
     "                              '<frame> mouse <dx> <dy>', '<frame> scroll <dy>' (default: a built-in tour).\n"
@@ -478,9 +480,21 @@ void run_viewer(const Args& a, Gpu& gpu, rm_cloud* cloud) {
         float pose[16];
         cam.pose(pose);
         const float fov = cam.zoom;
-        gpu.check(rm_render(gpu.ctx, cloud, pose, fov, kW, kH, rgba.as<float>(),
-                            hit.as<int32_t>()), "rm_render");
-        gpu.check(rm_ctx_synchronize(gpu.ctx), "rm_render");
+        if (a.plain) {
+            gpu.check(rm_render(gpu.ctx, cloud, pose, fov, kW, kH, rgba.as<float>(),
+                                hit.as<int32_t>()), "rm_render");
+        } else {
+            rm_view v{};
+            std::memcpy(v.pose, pose, sizeof(v.pose));
+            v.fov_radians = fov;
+            v.width = kW;
+            v.height = kH;
+            v.offset = 0;
+            gpu.check(rm_render_batch_noalloc(gpu.ctx, cloud, &v, 1, rgba.as<float>(),
+                                              hit.as<int32_t>()),
+                      "rm_render_batch_noalloc");
+        }
+        gpu.check(rm_ctx_synchronize(gpu.ctx), "render");
         const bool dump = !a.dump_dir.empty() &&
                           (a.dump_every > 0 ? f % a.dump_every == 0 : f == frames - 1);
         if (dump) {

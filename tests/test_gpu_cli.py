@@ -203,7 +203,8 @@ def test_viewer_renders_the_scripted_path_and_counts_fps(rm_ctx, tmp_path):
     centred on the origin (the camera starts at (0, 0, 3) looking down -z), a scripted path of
     1500 frames at 1024x768: every dumped frame equals rm_render of the dumped pose and field of
     view on every pixel, the poses are the camera's (rmh_camera_poses of the same script), and
-    "<n> FPS" is printed once per second of the run."""
+    "<n> FPS" is printed once per second of the run; with -b 0 (the plain kernel) the frames are
+    byte-identical."""
     import re
     import torch
     from renderer_ray_marching_amd import api
@@ -244,3 +245,13 @@ def test_viewer_renders_the_scripted_path_and_counts_fps(rm_ctx, tmp_path):
         assert np.array_equal(ref.cpu().numpy(), frame), f
         hits += int((hit >= 0).sum().item())
     assert hits > 100_000
+    # -b 0: the frames come from the plain kernel instead (the default is the batched kernel
+    # through the allocation-free entry point); the same images
+    dump0 = tmp_path / "frames_plain"
+    r0 = _run(["-f", str(ply), "-r", "0.1", "--script", str(tmp_path / "path.txt"), "--frames",
+               "260", "--dt", "0.01", "--dump", str(dump0), "--dump-every", "250", "-b", "0"],
+              tmp_path)
+    assert r0.returncode == 0
+    for f in (0, 250):
+        assert (dump0 / f"frame_{f:05d}.npy").read_bytes() == \
+            (dump / f"frame_{f:05d}.npy").read_bytes(), f

@@ -56,6 +56,27 @@ def test_noalloc_equals_render_batch_on_two_calls(rm_ctx):
     a = _render(L.rm_render_batch, rm_ctx, mt, v)
     b = _render(L.rm_render_batch_noalloc, rm_ctx, mt, v)
     assert (a[1] == b[1]).all() and np.array_equal(a[0], b[0])
+    # a box scene seen from outside: rays that leave the cloud end at the boxes of the default
+    # early-miss frontier, kept in the same array; equal to the plain kernel on every pixel
+    print("case: noalloc vs the plain kernel, 30k-point box scene seen from outside", flush=True)
+    box = scenes.box_scene(30_000, seed=9)
+    p = torch.from_numpy(box.xyzw()).cuda()
+    tree = api.KDTreeFlann(rm_ctx)
+    tree.Build(p)
+    _, rc = tree.radii(0.05)
+    mb = api.PointcloudRayMarcher(p, torch.from_numpy(box.colors4()).cuda(), rc, tree)
+    vs = [scenes.box_scene_view(box, 96, 72, 80.0, distance=4.0, seed=s) for s in (1, 2)]
+    rgba, hit = _render(L.rm_render_batch_noalloc, rm_ctx, mb, vs)
+    off = 0
+    for view in vs:
+        n = view.width * view.height
+        prgba, phit = mb.render_to_texture(view.pose_colmajor(), view.fov, view.width,
+                                           view.height)
+        rm_ctx.sync()
+        assert (hit[off:off + n] == phit.cpu().numpy().ravel()).all()
+        assert np.array_equal(rgba[off:off + n], prgba.cpu().numpy().reshape(-1, 4))
+        off += n
+    assert (hit < 0).mean() > 0.2
 
 
 def test_noalloc_limits(rm_ctx):
