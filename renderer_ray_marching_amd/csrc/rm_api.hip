@@ -1012,7 +1012,7 @@ int rm_batch_autotune(rm_batch* b, int sample_stride, rm_tune* chosen) {
         b->tune = t;
         if (int r = batch_layout(b, sample_stride)) return r;
         float best = 1e30f;
-        for (int rep = 0; rep < 3; ++rep) {
+        for (int rep = 0; rep < 5; ++rep) {
             RM_HIP(ctx, hipEventRecord(e0, ctx->stream));
             if (int r = batch_enqueue(b, ctx->stream)) return r;
             RM_HIP(ctx, hipEventRecord(e1, ctx->stream));
