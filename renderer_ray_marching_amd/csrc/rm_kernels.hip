@@ -93,9 +93,75 @@ struct TopK {
             if (!dup) insert_sorted(kk);
         }
     }
+    // The same insertion for K = 10 as a top-down chain of exec-masked moves. v_cmpx leaves in
+    // exec the lanes whose kk goes below slot j, a subset of those below slot j + 1 (the list
+    // is ascending); they move slot j up one place and write kk into slot j, and the chain ends
+    // as soon as no lane goes further down. A lane for which kk >= key[9], or (NODUP) whose
+    // list holds kk's index, takes no part. Three vector instructions per slot moved, against
+    // K compares and 4 (K-1) + 2 selects for every call of insert_sorted. exec is restored on
+    // every path; vcc is clobbered.
+    template <bool NODUP>
+    __device__ __forceinline__ void insert_chain(uint64_t kk) {
+        static_assert(K == 10, "the chain is written for ten slots");
+        uint64_t saved;
+#define RM_CHAIN_STEP(hi, lo)                              \
+    "v_cmpx_lt_u64_e32 vcc, %[k], %[l" #lo "]\n"           \
+    "s_cbranch_execz 1f\n"                                 \
+    "v_mov_b64_e32 %[l" #hi "], %[l" #lo "]\n"             \
+    "v_mov_b64_e32 %[l" #lo "], %[k]\n"
+        if (NODUP) {
+            asm volatile(
+                "s_mov_b64 %[sv], exec\n"
+                "v_cmpx_lt_u64_e32 vcc, %[k], %[l9]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i0]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i1]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i2]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i3]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i4]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i5]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i6]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i7]\n"
+                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i8]\n"
+                "s_cbranch_execz 1f\n"
+                "v_mov_b64_e32 %[l9], %[k]\n"
+                RM_CHAIN_STEP(9, 8) RM_CHAIN_STEP(8, 7) RM_CHAIN_STEP(7, 6) RM_CHAIN_STEP(6, 5)
+                RM_CHAIN_STEP(5, 4) RM_CHAIN_STEP(4, 3) RM_CHAIN_STEP(3, 2) RM_CHAIN_STEP(2, 1)
+                RM_CHAIN_STEP(1, 0)
+                "1:\n"
+                "s_mov_b64 exec, %[sv]\n"
+                : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1]), [l2] "+v"(key[2]),
+                  [l3] "+v"(key[3]), [l4] "+v"(key[4]), [l5] "+v"(key[5]), [l6] "+v"(key[6]),
+                  [l7] "+v"(key[7]), [l8] "+v"(key[8]), [l9] "+v"(key[9])
+                : [k] "v"(kk), [ki] "v"(key_idx(kk)), [i0] "v"(key_idx(key[0])),
+                  [i1] "v"(key_idx(key[1])), [i2] "v"(key_idx(key[2])),
+                  [i3] "v"(key_idx(key[3])), [i4] "v"(key_idx(key[4])),
+                  [i5] "v"(key_idx(key[5])), [i6] "v"(key_idx(key[6])),
+                  [i7] "v"(key_idx(key[7])), [i8] "v"(key_idx(key[8]))
+                : "vcc");
+        } else {
+            asm volatile(
+                "s_mov_b64 %[sv], exec\n"
+                "v_cmpx_lt_u64_e32 vcc, %[k], %[l9]\n"
+                "s_cbranch_execz 1f\n"
+                "v_mov_b64_e32 %[l9], %[k]\n"
+                RM_CHAIN_STEP(9, 8) RM_CHAIN_STEP(8, 7) RM_CHAIN_STEP(7, 6) RM_CHAIN_STEP(6, 5)
+                RM_CHAIN_STEP(5, 4) RM_CHAIN_STEP(4, 3) RM_CHAIN_STEP(3, 2) RM_CHAIN_STEP(2, 1)
+                RM_CHAIN_STEP(1, 0)
+                "1:\n"
+                "s_mov_b64 exec, %[sv]\n"
+                : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1]), [l2] "+v"(key[2]),
+                  [l3] "+v"(key[3]), [l4] "+v"(key[4]), [l5] "+v"(key[5]), [l6] "+v"(key[6]),
+                  [l7] "+v"(key[7]), [l8] "+v"(key[8]), [l9] "+v"(key[9])
+                : [k] "v"(kk)
+                : "vcc");
+        }
+#undef RM_CHAIN_STEP
+    }
     template <bool NODUP>
     __device__ __forceinline__ void add(uint64_t kk) {
-        if (NODUP)
+        if constexpr (K == 10)
+            insert_chain<NODUP>(kk);
+        else if (NODUP)
             insert_nodup(kk);
         else
             insert(kk);
