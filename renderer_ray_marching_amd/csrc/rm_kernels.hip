@@ -262,7 +262,11 @@ __device__ __forceinline__ int child_entry(int ni, int s, int ref, int cnt) {
 // ------------------------------------------------------------------------------------------
 // amask: ballot of the active lanes (a wave-uniform 64-bit mask). A compare's ballot masked with
 // it needs no per-lane boolean (ballot of (active && cond) would materialise one in a VGPR).
-template <int K, bool CULL = false>
+// MAYBE_EMPTY: the node may have an empty child slot (count < 0). Without culling that is only
+// the root of a one-leaf tree, whose walk never expands another node, so the walk's loop skips
+// the test. The one or two entries are written with one two-word store and sp moves by the
+// number of children entered (near child on top).
+template <int K, bool CULL = false, bool MAYBE_EMPTY = CULL>
 __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float qx, float qy,
                                                      float qz, uint64_t amask, const TopK<K>& tk,
                                                      int* wstk, int& sp, int* err,
@@ -280,24 +284,30 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
     const float w = tk.worst_d2();
     const float m0 = mindist(f[0], f[1], f[2], f[3], f[4], f[5], qx, qy, qz);
     const float m1 = mindist(f[6], f[7], f[8], f[9], f[10], f[11], qx, qy, qz);
-    const uint64_t b0 = c0 >= 0 ? __builtin_amdgcn_ballot_w64(m0 <= w) & amask : 0ull;
-    const uint64_t b1 = c1 >= 0 ? __builtin_amdgcn_ballot_w64(m1 <= w) & amask : 0ull;
+    uint64_t b0 = __builtin_amdgcn_ballot_w64(m0 <= w) & amask;
+    uint64_t b1 = __builtin_amdgcn_ballot_w64(m1 <= w) & amask;
+    if (CULL || MAYBE_EMPTY) {
+        b0 = c0 >= 0 ? b0 : 0ull;
+        b1 = c1 >= 0 ? b1 : 0ull;
+    }
+    if ((b0 | b1) == 0) return;
     if (sp + 2 > kWaveStack) {
-        if (b0 | b1) *err = 1;
+        *err = 1;
         return;
     }
     const int e0 = child_entry(ni, 0, r0, c0), e1 = child_entry(ni, 1, r1, c1);
+    int lo = b0 ? e0 : e1, hi = lo, n = 1;
     if (b0 && b1) {
         // near child (popped first) = the one nearer for most active lanes
         const uint64_t pref0 = __builtin_amdgcn_ballot_w64(m0 <= m1) & amask;
         const bool near0 = 2 * __popcll(pref0) >= __popcll(amask);
-        wstk[sp++] = near0 ? e1 : e0;
-        wstk[sp++] = near0 ? e0 : e1;
-    } else if (b0) {
-        wstk[sp++] = e0;
-    } else if (b1) {
-        wstk[sp++] = e1;
+        lo = near0 ? e1 : e0;
+        hi = near0 ? e0 : e1;
+        n = 2;
     }
+    wstk[sp] = lo;
+    wstk[sp + 1] = hi;
+    sp += n;
 }
 
 // WIDE: a leaf of exactly kWideLeaf points (every leaf of a leaf-16 tree but the last one) is
@@ -385,7 +395,7 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
     CFloat* pf = (CFloat*)t.pts;
     const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
     int sp = 0;
-    packet_push_children<K>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err);
+    packet_push_children<K, false, true>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err);
     while (sp > 0) {
         --sp;
         const int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
@@ -420,7 +430,7 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
     CUint* nbits = CULL ? mask + cull_mask_words(t.n) : nullptr;
     const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
     int sp = 0;
-    packet_push_children<K, CULL>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err, nbits);
+    packet_push_children<K, CULL, true>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err, nbits);
     while (sp > 0) {
         int nq = 0;
         while (sp > 0 && nq < kLeafQueue) {
