@@ -61,7 +61,9 @@ float rm_ctx_last_kernel_ms(const rm_ctx* ctx);
  * 16 B). Replaces kdtree::KDTreeFlann::Build (src/kdtree/kdtree_flann.cu:101-117) and
  * KDTreeCuda3dIndex::buildIndex (third_party/flann/algorithms/kdtree_cuda_3d_index.h:124-141).
  * leaf_max: points per leaf (the reference uses 64, kdtree_cuda_3d_index.h:55-61; 0 picks
- * the library default). The tree copies the points; the input may be freed afterwards. */
+ * the library default). The tree copies the points; the input may be freed afterwards.
+ * Trees of more than 2^26 nodes (over a billion points at the default 16 per leaf, 2^26 points
+ * at one per leaf) are refused with RM_ERR_UNSUPPORTED. */
 int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tree** out);
 int rm_tree_destroy(rm_tree* tree);
 int rm_tree_info(const rm_tree* tree, int64_t* n, int64_t* n_nodes, int* depth, int* leaf_max);

@@ -19,12 +19,12 @@ constexpr int kMarchK = 10;
 
 // Largest tree depth the traversals accept (stack bound); the host refuses deeper trees.
 constexpr int kMaxDepth = 60;
+// Most node records a tree may have: the packet walks address a record by a 32-bit byte
+// offset (64 B each, under 4 GiB). 2^26 records hold over a billion points at 16 per leaf.
+constexpr int64_t kMaxNodes = int64_t(1) << 26;
 constexpr int kWaveStack = 64;
-// Leaves queued per round of the batched march's packet traversal.
-#ifndef RM_LEAF_QUEUE
-#define RM_LEAF_QUEUE 2
-#endif
-constexpr int kLeafQueue = RM_LEAF_QUEUE;
+// Leaves queued per round of the batched march's packet traversal (two scalar registers).
+constexpr int kLeafQueue = 2;
 // Early-miss frontier of the batched march: entries of this many floats, depth at most this.
 constexpr int kFrontierFloats = 8;
 constexpr int kMaxFrontierDepth = 10;

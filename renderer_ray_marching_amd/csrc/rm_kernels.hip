@@ -260,6 +260,13 @@ __device__ __forceinline__ int child_entry(int ni, int s, int ref, int cnt) {
 // leaf points are read with scalar loads (the node index is wave-uniform) and broadcast to
 // every lane. The stack is wave-uniform and lives in LDS (one row of kWaveStack per wave).
 // ------------------------------------------------------------------------------------------
+// Node record ni of a packet walk (wave-uniform index): a 32-bit byte offset (trees have at most
+// kMaxNodes records), which the scalar load takes in one register.
+__device__ __forceinline__ CFloat* node_rec(CFloat* nf, int ni) {
+    typedef const __attribute__((address_space(4))) char CChar;
+    return (CFloat*)((CChar*)nf + ((uint32_t)ni << 6));
+}
+
 // amask: ballot of the active lanes (a wave-uniform 64-bit mask). A compare's ballot masked with
 // it needs no per-lane boolean (ballot of (active && cond) would materialise one in a VGPR).
 // MAYBE_EMPTY: the node may have an empty child slot (count < 0). Without culling that is only
@@ -271,7 +278,7 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
                                                      float qz, uint64_t amask, const TopK<K>& tk,
                                                      int* wstk, int& sp, int* err,
                                                      CUint* nbits = nullptr) {
-    CFloat* f = nf + (int64_t)ni * 16;
+    CFloat* f = node_rec(nf, ni);
     CInt* fi = (CInt*)f;
     const int r0 = fi[12], r1 = fi[13];
     int c0 = fi[14], c1 = fi[15];
@@ -401,7 +408,7 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
         const int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
         if (e & 1) {
             const int ni = e >> 2, s = (e >> 1) & 1;
-            CFloat* f = nf + (int64_t)ni * 16;
+            CFloat* f = node_rec(nf, ni);
             CInt* fi = (CInt*)f;
             CFloat* bx = f + 6 * s;
             const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
@@ -423,7 +430,7 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
 // entries still on the stack when they are expanded.
 template <int K, bool NODUP = false, bool CULL = false>
 __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz, bool active,
-                                  TopK<K>& tk, int* wstk, int* lq, int* err,
+                                  TopK<K>& tk, int* wstk, int* err,
                                   CUint* mask = nullptr) {
     CFloat* nf = (CFloat*)t.nodes;
     CFloat* pf = (CFloat*)t.pts;
@@ -431,21 +438,27 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
     const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
     int sp = 0;
     packet_push_children<K, CULL, true>(nf, 0, qx, qy, qz, amask, tk, wstk, sp, err, nbits);
+    static_assert(kLeafQueue == 2, "the leaf queue is two scalar registers");
     while (sp > 0) {
-        int nq = 0;
+        int nq = 0, q0 = 0, q1 = 0;
         while (sp > 0 && nq < kLeafQueue) {
             --sp;
             const int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
-            if (e & 1)
-                lq[nq++] = e;
-            else
+            if (e & 1) {
+                if (nq == 0)
+                    q0 = e;
+                else
+                    q1 = e;
+                ++nq;
+            } else {
                 packet_push_children<K, CULL>(nf, e >> 1, qx, qy, qz, amask, tk, wstk, sp, err,
                                               nbits);
+            }
         }
         for (int i = 0; i < nq; ++i) {
-            const int e = __builtin_amdgcn_readfirstlane(lq[i]);
+            const int e = i == 0 ? q0 : q1;
             const int ni = e >> 2, s = (e >> 1) & 1;
-            CFloat* f = nf + (int64_t)ni * 16;
+            CFloat* f = node_rec(nf, ni);
             CInt* fi = (CInt*)f;
             CFloat* bx = f + 6 * s;
             const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
@@ -1059,7 +1072,6 @@ template <int MODE, bool WARM, int WPB, bool CULL, bool OVF = false>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) void batch_kernel(
     BatchParams p, int maxsp) {
     __shared__ int wstk[WPB][kWaveStack];
-    __shared__ int lq[WPB][kLeafQueue];
     __shared__ RayStateLds<WPB> rs;
     extern __shared__ int lstk[];
     const int w = threadIdx.x >> 6;
@@ -1166,7 +1178,7 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
                 // a fresh lane's pads (index 0xffffffff) never match a real candidate, so the
                 // duplicate check is harmless for it and one traversal serves both kinds
                 knn_packet_rounds<kMarchK, WARM, CULL>(p.cloud.tree, px, py, pz, has, tk, wstk[w],
-                                                       lq[w], p.err, (CUint*)vmask);
+                                                       p.err, (CUint*)vmask);
             } else if (has) {
                 // the first lane_lds entries of the lane's stack in LDS, deeper ones in its
                 // column of the overflow area (never reached when lane_lds == maxsp)
