@@ -168,32 +168,12 @@ struct TopK {
     }
 };
 
-__device__ __forceinline__ void cmpx(uint64_t& a, uint64_t& b) {
-    const uint64_t lo = a < b ? a : b;
-    const uint64_t hi = a < b ? b : a;
-    a = lo;
-    b = hi;
-}
-
-// Sorts 10 keys: Batcher's odd-even merge sort network for 16 inputs restricted to the first 10
-// (the other six act as +infinity and never move); 32 compare-exchanges, checked exhaustively
-// with the 0-1 principle.
-__device__ __forceinline__ void sort10(uint64_t* k) {
-    cmpx(k[0], k[1]); cmpx(k[2], k[3]); cmpx(k[0], k[2]); cmpx(k[1], k[3]); cmpx(k[1], k[2]);
-    cmpx(k[4], k[5]); cmpx(k[6], k[7]); cmpx(k[4], k[6]); cmpx(k[5], k[7]); cmpx(k[5], k[6]);
-    cmpx(k[0], k[4]); cmpx(k[2], k[6]); cmpx(k[2], k[4]); cmpx(k[1], k[5]); cmpx(k[3], k[7]);
-    cmpx(k[3], k[5]); cmpx(k[1], k[2]); cmpx(k[3], k[4]); cmpx(k[5], k[6]); cmpx(k[8], k[9]);
-    cmpx(k[0], k[8]); cmpx(k[4], k[8]); cmpx(k[2], k[4]); cmpx(k[6], k[8]); cmpx(k[1], k[9]);
-    cmpx(k[5], k[9]); cmpx(k[3], k[5]); cmpx(k[7], k[9]); cmpx(k[1], k[2]); cmpx(k[3], k[4]);
-    cmpx(k[5], k[6]); cmpx(k[7], k[8]);
-}
-
 // Sorts 10 keys by insertion, each key moved down an exec-masked chain as in
 // TopK::insert_chain: key j is held in t, and while some lane's t is below slot i (v_cmpx),
 // those lanes move slot i up and write t into slot i. Cost per key: 2 + 3 x (places the
 // furthest lane moves it). For the warm start's seed (the previous step's ten nearest,
 // re-measured from a point a little further along the ray, so nearly in order) that is about
-// 80 vector instructions per wavefront step against 160 for sort10.
+// 80 vector instructions per wavefront step against 160 for a 32-exchange sorting network.
 __device__ __forceinline__ void sort10_nearly_sorted(uint64_t* k) {
     uint64_t t, saved;
 #define RM_SORT_STEP(hi, lo)                               \
@@ -1101,7 +1081,6 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(8))) v
         }
         CFloat* V = vf + (int64_t)lo * VSTRIDE;
         CInt* VI = (CInt*)V;
-        const float pane = V[16];
         const int W = VI[17], H = VI[18], tiles_x = VI[19];
         const int64_t pix_off =
             *(const __attribute__((address_space(4))) int64_t*)(V + offsetof(DevView, pix_offset) / 4);
