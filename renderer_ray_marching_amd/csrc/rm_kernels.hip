@@ -261,19 +261,19 @@ __device__ __forceinline__ void packet_push_children(CFloat* nf, int ni, float q
     CFloat* f = node_rec(nf, ni);
     CInt* fi = (CInt*)f;
     const int r0 = fi[12], r1 = fi[13];
-    int c0 = fi[14], c1 = fi[15];
-    if (CULL) {
-        // a child slot with no point inside the view frustum is treated as empty
-        const uint32_t nb = nbits[ni >> 2] >> ((ni & 3) * 8);
-        if (!(nb & 1u)) c0 = -1;
-        if (!(nb & 2u)) c1 = -1;
-    }
+    const int c0 = fi[14], c1 = fi[15];
     const float w = tk.worst_d2();
     const float m0 = mindist(f[0], f[1], f[2], f[3], f[4], f[5], qx, qy, qz);
     const float m1 = mindist(f[6], f[7], f[8], f[9], f[10], f[11], qx, qy, qz);
     uint64_t b0 = __builtin_amdgcn_ballot_w64(m0 <= w) & amask;
     uint64_t b1 = __builtin_amdgcn_ballot_w64(m1 <= w) & amask;
-    if (CULL || MAYBE_EMPTY) {
+    if (CULL) {
+        // a child slot with no point inside the view frustum is not entered (an empty slot
+        // has its bit clear too)
+        const uint32_t nb = nbits[ni >> 2] >> ((ni & 3) * 8);
+        b0 = (nb & 1u) ? b0 : 0ull;
+        b1 = (nb & 2u) ? b1 : 0ull;
+    } else if (MAYBE_EMPTY) {
         b0 = c0 >= 0 ? b0 : 0ull;
         b1 = c1 >= 0 ? b1 : 0ull;
     }

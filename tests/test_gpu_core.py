@@ -75,21 +75,25 @@ def test_knn_ties_duplicates_and_padding(rm_ctx):
     p = np.concatenate([p, p[:20]])                       # exact duplicates
     p = p[np.random.default_rng(0).permutation(p.shape[0])].astype(np.float32)
     q = np.concatenate([p[:60], p[:60] + np.float32(h / 2)]).astype(np.float32)
-    for leaf in (1, 4, 16):
-        t = _tree(rm_ctx, p, leaf)
-        _, idx, d2 = t.SearchKNN(_dev(q), 12)
-        rm_ctx.sync()
-        oi, od = ob.knn_brute(p, q, 12)
-        assert (idx.cpu().numpy() == oi).all() and (d2.cpu().numpy() == od).all()
+    # k = 10 runs the exec-masked insertion chain (every other k the select network); lattice
+    # queries tie on d2 everywhere, so the order rests on the index half of the keys
+    for k in (10, 12):
+        for leaf in (1, 4, 16):
+            t = _tree(rm_ctx, p, leaf)
+            _, idx, d2 = t.SearchKNN(_dev(q), k)
+            rm_ctx.sync()
+            oi, od = ob.knn_brute(p, q, k)
+            assert (idx.cpu().numpy() == oi).all() and (d2.cpu().numpy() == od).all(), (k, leaf)
     # fewer points than k: -1 / +INF padding
     small = p[:5]
     t = _tree(rm_ctx, small)
-    _, idx, d2 = t.SearchKNN(_dev(q[:10]), 8)
-    rm_ctx.sync()
-    idx, d2 = idx.cpu().numpy(), d2.cpu().numpy()
-    assert (idx[:, 5:] == -1).all() and np.isinf(d2[:, 5:]).all()
-    oi, od = ob.knn_brute(small, q[:10], 8)
-    assert (idx == oi).all() and (d2 == od).all()
+    for k in (8, 10):
+        _, idx, d2 = t.SearchKNN(_dev(q[:10]), k)
+        rm_ctx.sync()
+        idx, d2 = idx.cpu().numpy(), d2.cpu().numpy()
+        assert (idx[:, 5:] == -1).all() and np.isinf(d2[:, 5:]).all()
+        oi, od = ob.knn_brute(small, q[:10], k)
+        assert (idx == oi).all() and (d2 == od).all(), k
 
 
 def test_empty_queries_and_empty_cloud(rm_ctx):
