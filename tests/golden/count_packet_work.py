@@ -62,6 +62,12 @@ def count(name, stride, check_every):
                 "candidates": round(tot["lane_candidates"] / tot["lane_steps"], 2),
                 "inserts": round(tot["lane_inserts"] / tot["lane_steps"], 2)},
             "steps_per_ray": round(tot["lane_steps"] / tot["rays"], 2),
+            "insertion_chain": {
+                "slots_per_wave_insert": round(tot["insert_slots"] / max(1, tot["wave_inserts"]), 2),
+                "wave_inserts_moving_nothing": round(tot["inserts_no_move"] /
+                                                     max(1, tot["wave_inserts"]), 3),
+                "seed_moves_per_warm_wave_step": round(tot["seed_moves"] /
+                                                       max(1, tot["seed_steps"]), 2)},
             "containing_depth_mean": round(tot["containing_depth"] /
                                            max(1, tot["containing_steps"]), 2),
             "pixels_checked_against_per_lane_model": checked,

@@ -72,10 +72,12 @@ inline float mindist(const float* b, float qx, float qy, float qz) {
 struct TopK {
     int k;
     uint64_t key[32];
+    int moved = 0;   // slots the last insert wrote (0: not inserted)
     void init(int kk, uint64_t pad) { k = kk; for (int j = 0; j < k; ++j) key[j] = pad; }
     float worst() const { return key_d2(key[k - 1]); }
     // true when the list changed
     bool insert(uint64_t kk, bool nodup) {
+        moved = 0;
         if (!(kk < key[k - 1])) return false;
         if (nodup)
             for (int j = 0; j < k; ++j)
@@ -83,6 +85,7 @@ struct TopK {
         int pos = k - 1;
         while (pos > 0 && kk < key[pos - 1]) { key[pos] = key[pos - 1]; --pos; }
         key[pos] = kk;
+        moved = k - pos;
         return true;
     }
 };
@@ -314,10 +317,15 @@ void wm_render(const void* t, const float* pose, float pane, int W, int H, const
     std::memcpy(out, &total, sizeof(Counts));
 }
 
+// insert_slots: over the wavefront inserts, the slots written by the lane that writes most
+// (what the kernel's exec-masked insertion chain runs for); inserts_no_move: wavefront inserts
+// in which no lane's list changes; seed_moves: over the warm-started wavefront steps, the sum
+// over keys j of the places the furthest-moving lane moves key j when the re-measured seed is
+// sorted by insertion (the kernel's chain sort), seed_steps: those wavefront steps.
 typedef struct wm_packet_counts {
     uint64_t wave_steps, lane_steps, expansions, retests, scans, candidates, wave_inserts,
         lane_inserts, lane_child_passes, lane_candidates, rays, hits, tiles, containing_depth,
-        containing_steps;
+        containing_steps, insert_slots, inserts_no_move, seed_moves, seed_steps;
 } wm_packet_counts;
 
 // Tiles tile_stride * j of one view (8x8, row-major); hit of each counted pixel (others keep
@@ -360,7 +368,8 @@ void wm_render_packet(const void* t, const float* pose, float pane, int W, int H
                 if (na == 0) break;
                 c.wave_steps++;
                 c.lane_steps += (uint64_t)na;
-                bool all_warm = true;
+                bool all_warm = true, any_warm = false;
+                int seed_mv[kMarchK] = {};
                 for (int l = 0; l < 64; ++l) {
                     if (!has[l]) continue;
                     px[l] = ray[l].ox + ray[l].dx * tt[l];
@@ -373,6 +382,12 @@ void wm_render_packet(const void* t, const float* pose, float pane, int W, int H
                             const float* q = T->xyzr.data() + 4 * (size_t)id;
                             tk[l].key[k] = make_key(dist2(q[0], q[1], q[2], px[l], py[l], pz[l]), id);
                         }
+                        any_warm = true;
+                        for (int j = 1; j < kMarchK; ++j) {
+                            int m = 0;
+                            for (int i = 0; i < j; ++i) m += tk[l].key[j] < tk[l].key[i];
+                            seed_mv[j] = std::max(seed_mv[j], m);
+                        }
                         std::sort(tk[l].key, tk[l].key + kMarchK);
                         if (bound_only) {
                             tk[l].init(kMarchK, tk[l].key[kMarchK - 1] | 0xffffffffull);
@@ -382,6 +397,10 @@ void wm_render_packet(const void* t, const float* pose, float pane, int W, int H
                         tk[l].init(kMarchK, kPadKey);
                         all_warm = false;
                     }
+                }
+                if (any_warm) {
+                    c.seed_steps++;
+                    for (int j = 1; j < kMarchK; ++j) c.seed_moves += (uint64_t)seed_mv[j];
                 }
                 if (all_warm) {
                     // deepest node whose box holds every lane's ball with a float margin
@@ -472,11 +491,15 @@ void wm_render_packet(const void* t, const float* pose, float pane, int W, int H
                             }
                             if (!pass) continue;
                             c.wave_inserts++;
+                            int mv = 0;
                             for (int l = 0; l < 64; ++l) {
                                 if (!has[l]) continue;
                                 c.lane_inserts += tk[l].insert(make_key(d[l], fbits(p[4 * j + 3])),
                                                                nodup[l]);
+                                mv = std::max(mv, tk[l].moved);
                             }
+                            c.insert_slots += (uint64_t)mv;
+                            c.inserts_no_move += mv == 0;
                         }
                     }
                 }

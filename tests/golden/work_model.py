@@ -35,7 +35,8 @@ class PacketCounts(C.Structure):
     _fields_ = [(f, C.c_uint64) for f in
                 ("wave_steps", "lane_steps", "expansions", "retests", "scans", "candidates",
                  "wave_inserts", "lane_inserts", "lane_child_passes", "lane_candidates", "rays",
-                 "hits", "tiles", "containing_depth", "containing_steps")]
+                 "hits", "tiles", "containing_depth", "containing_steps", "insert_slots",
+                 "inserts_no_move", "seed_moves", "seed_steps")]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
