@@ -324,26 +324,20 @@ __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, flo
         }
         return;
     }
-    // culling: a leaf of at most 32 points takes its mask bits from one two-word window (the
-    // mask array ends with a spare pair of words, so the second word is always there)
-    uint32_t leaf_keep = 0xffffffffu;
-    const bool one_window = CULL && cnt <= 32;
-    if (one_window) {
+    // culling: a leaf (at most 64 points) takes its mask bits from one three-word window,
+    // shifted to start at its first point (the mask array ends with a spare pair of words, so
+    // the window is always inside it)
+    uint64_t leaf_keep = ~0ull;
+    if (CULL) {
         const int64_t w0 = (int64_t)b >> 5;
-        const uint64_t win = (uint64_t)mask[w0] | ((uint64_t)mask[w0 + 1] << 32);
-        leaf_keep = (uint32_t)(win >> (b & 31));
+        const uint32_t sh = (uint32_t)b & 31u;
+        const uint64_t w01 = (uint64_t)mask[w0] | ((uint64_t)mask[w0 + 1] << 32);
+        leaf_keep = sh ? (w01 >> sh) | ((uint64_t)mask[w0 + 2] << (64u - sh)) : w01;
     }
     int j = 0;
     for (; j + 4 <= cnt; j += 4) {
         // culled points (wave-uniform bits) are skipped by the scalar unit
-        uint32_t keep = 0xfu;
-        if (CULL && one_window) {
-            keep = (leaf_keep >> j) & 0xfu;
-        } else if (CULL) {
-            const int64_t i0 = (int64_t)b + j;
-            const uint64_t win = (uint64_t)mask[i0 >> 5] | ((uint64_t)mask[(i0 >> 5) + 1] << 32);
-            keep = (uint32_t)(win >> (i0 & 31)) & 0xfu;
-        }
+        const uint32_t keep = CULL ? (uint32_t)(leaf_keep >> j) & 0xfu : 0xfu;
         float x[4], y[4], z[4];
         uint32_t id[4];
 #pragma unroll
@@ -364,7 +358,7 @@ __device__ __forceinline__ void packet_scan_leaf(CFloat* pf, int b, int cnt, flo
                 tk.template add<NODUP>(make_key(d[u], id[u]));
     }
     for (; j < cnt; ++j) {
-        if (CULL && !mask_bit(mask, (int64_t)b + j)) continue;
+        if (CULL && !((leaf_keep >> j) & 1u)) continue;
         float d = dist2(p[4 * j], p[4 * j + 1], p[4 * j + 2], qx, qy, qz);
         tk.template add<NODUP>(make_key(d, __float_as_uint(p[4 * j + 3])));
     }
