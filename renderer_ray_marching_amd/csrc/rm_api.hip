@@ -381,8 +381,6 @@ int tree_build_host(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_
     if (!rm_build_host_tree(host.data(), n, leaf_max, ht, msg)) return fail(ctx, RM_ERR_INVALID, msg);
     if (ht.depth > rm::kMaxDepth)
         return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: tree deeper than supported");
-    if ((int64_t)ht.nodes.size() > rm::kMaxNodes)
-        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: more than 2^26 tree nodes");
     t->n_nodes = (int64_t)ht.nodes.size();
     t->depth = ht.depth;
     int nlev = 0;
@@ -423,8 +421,6 @@ int tree_build_device(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, r
     if (!rm_tree_shape(n, leaf_max, shape, msg)) return fail(ctx, RM_ERR_INVALID, msg);
     if (shape.depth > rm::kMaxDepth)
         return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: tree deeper than supported");
-    if ((int64_t)shape.nodes.size() > rm::kMaxNodes)
-        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: more than 2^26 tree nodes");
     const size_t nn = shape.nodes.size();
     t->n_nodes = (int64_t)nn;
     t->depth = shape.depth;
@@ -506,6 +502,9 @@ int rm_tree_build(rm_ctx* ctx, const float* xyzw, int64_t n, int leaf_max, rm_tr
     *out = nullptr;
     if (!xyzw || n <= 0) return fail(ctx, RM_ERR_INVALID, "rm_tree_build: empty input");
     if (leaf_max == 0) leaf_max = kDefaultLeaf;
+    // refused before any layout work: the packet walks address node records by 32-bit offsets
+    if (leaf_max >= 1 && leaf_max <= 64 && rm_tree_node_count(n, leaf_max) > rm::kMaxNodes)
+        return fail(ctx, RM_ERR_UNSUPPORTED, "rm_tree_build: more than 2^26 tree nodes");
     if (int r = set_device(ctx)) return r;
     rm_tree* t = new (std::nothrow) rm_tree();
     if (!t) return fail(ctx, RM_ERR_NOMEM, "host allocation failed");

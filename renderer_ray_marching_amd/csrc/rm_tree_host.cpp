@@ -182,6 +182,21 @@ bool rm_build_host_tree(const float* xyzw, int64_t n, int leaf_max, RmHostTree& 
     return true;
 }
 
+int64_t rm_tree_node_count(int64_t n, int leaf_max) {
+    if (n <= leaf_max) return 1;
+    std::unordered_map<int64_t, int64_t> memo;
+    auto inner = [&](auto&& self, int64_t c) -> int64_t {
+        if (c <= leaf_max) return 0;
+        auto it = memo.find(c);
+        if (it != memo.end()) return it->second;
+        const int64_t nl = split_left_count(c, leaf_max);
+        const int64_t v = 1 + self(self, nl) + self(self, c - nl);
+        memo[c] = v;
+        return v;
+    };
+    return inner(inner, n);
+}
+
 bool rm_tree_shape(int64_t n, int leaf_max, RmShape& out, std::string& err) {
     if (n <= 0) {
         err = "rm_tree_build: empty point set";

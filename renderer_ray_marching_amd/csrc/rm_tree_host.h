@@ -52,3 +52,8 @@ struct RmShape {
 };
 
 bool rm_tree_shape(int64_t n, int leaf_max, RmShape& out, std::string& err);
+
+// Node records of the tree of n >= 1 points with leaf size leaf_max in [1, 64] (its inner nodes,
+// or the one root record of a single leaf), without laying the tree out: subtrees of equal
+// counts have equal shapes, and the splits of one level produce at most two counts.
+int64_t rm_tree_node_count(int64_t n, int leaf_max);

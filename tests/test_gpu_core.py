@@ -96,6 +96,17 @@ def test_knn_ties_duplicates_and_padding(rm_ctx):
         assert (idx == oi).all() and (d2 == od).all(), k
 
 
+def test_tree_over_the_node_limit_is_refused(rm_ctx):
+    """Trees of more than 2^26 node records are refused before any layout work (the packet
+    walks address node records by 32-bit byte offsets): 2^26 + 2 points at one per leaf make
+    2^26 + 1 inner nodes."""
+    p = torch.zeros(((1 << 26) + 2, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(RMError, match=r"2\^26"):
+        api.KDTreeFlann(rm_ctx, leaf_max=1).Build(p)
+    del p
+    torch.cuda.empty_cache()
+
+
 def test_empty_queries_and_empty_cloud(rm_ctx):
     p = scenes.box_scene(2000, seed=3).xyzw()
     t = _tree(rm_ctx, p)
