@@ -157,10 +157,31 @@ struct TopK {
         }
 #undef RM_CHAIN_STEP
     }
+    // The chain for K = 2 (the radii pass): at most two exec-masked steps.
+    __device__ __forceinline__ void insert_chain2(uint64_t kk) {
+        static_assert(K == 2, "the chain is written for two slots");
+        uint64_t saved;
+        asm volatile(
+            "s_mov_b64 %[sv], exec\n"
+            "v_cmpx_lt_u64_e32 vcc, %[k], %[l1]\n"
+            "s_cbranch_execz 1f\n"
+            "v_mov_b64_e32 %[l1], %[k]\n"
+            "v_cmpx_lt_u64_e32 vcc, %[k], %[l0]\n"
+            "s_cbranch_execz 1f\n"
+            "v_mov_b64_e32 %[l1], %[l0]\n"
+            "v_mov_b64_e32 %[l0], %[k]\n"
+            "1:\n"
+            "s_mov_b64 exec, %[sv]\n"
+            : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1])
+            : [k] "v"(kk)
+            : "vcc");
+    }
     template <bool NODUP>
     __device__ __forceinline__ void add(uint64_t kk) {
         if constexpr (K == 10)
             insert_chain<NODUP>(kk);
+        else if constexpr (K == 2 && !NODUP)
+            insert_chain2(kk);
         else if (NODUP)
             insert_nodup(kk);
         else
