@@ -157,6 +157,35 @@ struct TopK {
         }
 #undef RM_CHAIN_STEP
     }
+    // The chain for K = 16 (radius searches of up to 16 neighbours, kNN with k = 11..16).
+    __device__ __forceinline__ void insert_chain16(uint64_t kk) {
+        static_assert(K == 16, "the chain is written for sixteen slots");
+        uint64_t saved;
+#define RM_CHAIN_STEP(hi, lo)                              \
+    "v_cmpx_lt_u64_e32 vcc, %[k], %[l" #lo "]\n"           \
+    "s_cbranch_execz 1f\n"                                 \
+    "v_mov_b64_e32 %[l" #hi "], %[l" #lo "]\n"             \
+    "v_mov_b64_e32 %[l" #lo "], %[k]\n"
+        asm volatile(
+            "s_mov_b64 %[sv], exec\n"
+            "v_cmpx_lt_u64_e32 vcc, %[k], %[l15]\n"
+            "s_cbranch_execz 1f\n"
+            "v_mov_b64_e32 %[l15], %[k]\n"
+            RM_CHAIN_STEP(15, 14) RM_CHAIN_STEP(14, 13) RM_CHAIN_STEP(13, 12)
+            RM_CHAIN_STEP(12, 11) RM_CHAIN_STEP(11, 10) RM_CHAIN_STEP(10, 9) RM_CHAIN_STEP(9, 8)
+            RM_CHAIN_STEP(8, 7) RM_CHAIN_STEP(7, 6) RM_CHAIN_STEP(6, 5) RM_CHAIN_STEP(5, 4)
+            RM_CHAIN_STEP(4, 3) RM_CHAIN_STEP(3, 2) RM_CHAIN_STEP(2, 1) RM_CHAIN_STEP(1, 0)
+            "1:\n"
+            "s_mov_b64 exec, %[sv]\n"
+            : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1]), [l2] "+v"(key[2]),
+              [l3] "+v"(key[3]), [l4] "+v"(key[4]), [l5] "+v"(key[5]), [l6] "+v"(key[6]),
+              [l7] "+v"(key[7]), [l8] "+v"(key[8]), [l9] "+v"(key[9]), [l10] "+v"(key[10]),
+              [l11] "+v"(key[11]), [l12] "+v"(key[12]), [l13] "+v"(key[13]),
+              [l14] "+v"(key[14]), [l15] "+v"(key[15])
+            : [k] "v"(kk)
+            : "vcc");
+#undef RM_CHAIN_STEP
+    }
     // The chain for K = 2 (the radii pass): at most two exec-masked steps.
     __device__ __forceinline__ void insert_chain2(uint64_t kk) {
         static_assert(K == 2, "the chain is written for two slots");
@@ -182,6 +211,8 @@ struct TopK {
             insert_chain<NODUP>(kk);
         else if constexpr (K == 2 && !NODUP)
             insert_chain2(kk);
+        else if constexpr (K == 16 && !NODUP)
+            insert_chain16(kk);
         else if (NODUP)
             insert_nodup(kk);
         else
