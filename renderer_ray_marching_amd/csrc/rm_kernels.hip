@@ -93,126 +93,104 @@ struct TopK {
             if (!dup) insert_sorted(kk);
         }
     }
-    // The same insertion for K = 10 as a top-down chain of exec-masked moves. v_cmpx leaves in
-    // exec the lanes whose kk goes below slot j, a subset of those below slot j + 1 (the list
-    // is ascending); they move slot j up one place and write kk into slot j, and the chain ends
-    // as soon as no lane goes further down. A lane for which kk >= key[9], or (NODUP) whose
-    // list holds kk's index, takes no part. Three vector instructions per slot moved, against
-    // K compares and 4 (K-1) + 2 selects for every call of insert_sorted. exec is restored on
-    // every path; vcc is clobbered.
+    // The same insertion as a top-down chain of exec-masked moves (GCN assembly; the list sizes
+    // the library instantiates). v_cmpx leaves in exec the lanes whose kk goes below slot j, a
+    // subset of those below slot j + 1 (the list is ascending); they move slot j up one place
+    // and write kk into slot j, and the chain ends as soon as no lane goes further down. A lane
+    // for which kk >= key[K-1], or (NODUP) whose list holds kk's index, takes no part. Three
+    // vector instructions per slot moved, against K compares and 4 (K-1) + 2 selects for every
+    // call of insert_sorted. exec is restored on every path; vcc is clobbered.
+#define RM_CS(hi, lo)                                      \
+    "v_cmpx_lt_u64_e32 vcc, %[k], %[l" #lo "]\n"           \
+    "s_cbranch_execz 1f\n"                                 \
+    "v_mov_b64_e32 %[l" #hi "], %[l" #lo "]\n"             \
+    "v_mov_b64_e32 %[l" #lo "], %[k]\n"
+#define RM_CS_4 RM_CS(3, 2) RM_CS(2, 1) RM_CS(1, 0)
+#define RM_CS_8 RM_CS(7, 6) RM_CS(6, 5) RM_CS(5, 4) RM_CS(4, 3) RM_CS_4
+#define RM_CS_10 RM_CS(9, 8) RM_CS(8, 7) RM_CS_8
+#define RM_CS_16                                                                               \
+    RM_CS(15, 14) RM_CS(14, 13) RM_CS(13, 12) RM_CS(12, 11) RM_CS(11, 10) RM_CS(10, 9) RM_CS_10
+#define RM_CS_32                                                                               \
+    RM_CS(31, 30) RM_CS(30, 29) RM_CS(29, 28) RM_CS(28, 27) RM_CS(27, 26) RM_CS(26, 25)        \
+    RM_CS(25, 24) RM_CS(24, 23) RM_CS(23, 22) RM_CS(22, 21) RM_CS(21, 20) RM_CS(20, 19)        \
+    RM_CS(19, 18) RM_CS(18, 17) RM_CS(17, 16) RM_CS(16, 15) RM_CS_16
+#define RM_OP(i) [l##i] "+v"(key[i])
+#define RM_OPS_2 RM_OP(0), RM_OP(1)
+#define RM_OPS_4 RM_OPS_2, RM_OP(2), RM_OP(3)
+#define RM_OPS_8 RM_OPS_4, RM_OP(4), RM_OP(5), RM_OP(6), RM_OP(7)
+#define RM_OPS_10 RM_OPS_8, RM_OP(8), RM_OP(9)
+#define RM_OPS_16 RM_OPS_10, RM_OP(10), RM_OP(11), RM_OP(12), RM_OP(13), RM_OP(14), RM_OP(15)
+#define RM_OPS_32                                                                              \
+    RM_OPS_16, RM_OP(16), RM_OP(17), RM_OP(18), RM_OP(19), RM_OP(20), RM_OP(21), RM_OP(22),    \
+        RM_OP(23), RM_OP(24), RM_OP(25), RM_OP(26), RM_OP(27), RM_OP(28), RM_OP(29), RM_OP(30), \
+        RM_OP(31)
+// the chain of a list of TOP + 1 slots; DUP: the duplicate test after the first compare
+#define RM_CHAIN(TOP, DUP, STEPS, OPS, ...)                                                    \
+    asm volatile("s_mov_b64 %[sv], exec\n"                                                     \
+                 "v_cmpx_lt_u64_e32 vcc, %[k], %[l" #TOP "]\n" DUP                             \
+                 "s_cbranch_execz 1f\n"                                                        \
+                 "v_mov_b64_e32 %[l" #TOP "], %[k]\n" STEPS                                    \
+                 "1:\n"                                                                        \
+                 "s_mov_b64 exec, %[sv]\n"                                                     \
+                 : [sv] "=&s"(saved), OPS                                                      \
+                 : [k] "v"(kk)__VA_ARGS__                                                      \
+                 : "vcc")
+    // the index of kk against slots 0..8 (slot 9 cannot hold it once kk < key[9])
+#define RM_DUP_10                                                                              \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i0]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i1]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i2]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i3]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i4]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i5]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i6]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i7]\n"                                                    \
+    "v_cmpx_ne_u32_e32 vcc, %[ki], %[i8]\n"
+    static constexpr bool kChain = K == 2 || K == 4 || K == 8 || K == 10 || K == 16 || K == 32;
     template <bool NODUP>
     __device__ __forceinline__ void insert_chain(uint64_t kk) {
-        static_assert(K == 10, "the chain is written for ten slots");
         uint64_t saved;
-#define RM_CHAIN_STEP(hi, lo)                              \
-    "v_cmpx_lt_u64_e32 vcc, %[k], %[l" #lo "]\n"           \
-    "s_cbranch_execz 1f\n"                                 \
-    "v_mov_b64_e32 %[l" #hi "], %[l" #lo "]\n"             \
-    "v_mov_b64_e32 %[l" #lo "], %[k]\n"
-        if (NODUP) {
-            asm volatile(
-                "s_mov_b64 %[sv], exec\n"
-                "v_cmpx_lt_u64_e32 vcc, %[k], %[l9]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i0]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i1]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i2]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i3]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i4]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i5]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i6]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i7]\n"
-                "v_cmpx_ne_u32_e32 vcc, %[ki], %[i8]\n"
-                "s_cbranch_execz 1f\n"
-                "v_mov_b64_e32 %[l9], %[k]\n"
-                RM_CHAIN_STEP(9, 8) RM_CHAIN_STEP(8, 7) RM_CHAIN_STEP(7, 6) RM_CHAIN_STEP(6, 5)
-                RM_CHAIN_STEP(5, 4) RM_CHAIN_STEP(4, 3) RM_CHAIN_STEP(3, 2) RM_CHAIN_STEP(2, 1)
-                RM_CHAIN_STEP(1, 0)
-                "1:\n"
-                "s_mov_b64 exec, %[sv]\n"
-                : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1]), [l2] "+v"(key[2]),
-                  [l3] "+v"(key[3]), [l4] "+v"(key[4]), [l5] "+v"(key[5]), [l6] "+v"(key[6]),
-                  [l7] "+v"(key[7]), [l8] "+v"(key[8]), [l9] "+v"(key[9])
-                : [k] "v"(kk), [ki] "v"(key_idx(kk)), [i0] "v"(key_idx(key[0])),
-                  [i1] "v"(key_idx(key[1])), [i2] "v"(key_idx(key[2])),
-                  [i3] "v"(key_idx(key[3])), [i4] "v"(key_idx(key[4])),
-                  [i5] "v"(key_idx(key[5])), [i6] "v"(key_idx(key[6])),
-                  [i7] "v"(key_idx(key[7])), [i8] "v"(key_idx(key[8]))
-                : "vcc");
-        } else {
-            asm volatile(
-                "s_mov_b64 %[sv], exec\n"
-                "v_cmpx_lt_u64_e32 vcc, %[k], %[l9]\n"
-                "s_cbranch_execz 1f\n"
-                "v_mov_b64_e32 %[l9], %[k]\n"
-                RM_CHAIN_STEP(9, 8) RM_CHAIN_STEP(8, 7) RM_CHAIN_STEP(7, 6) RM_CHAIN_STEP(6, 5)
-                RM_CHAIN_STEP(5, 4) RM_CHAIN_STEP(4, 3) RM_CHAIN_STEP(3, 2) RM_CHAIN_STEP(2, 1)
-                RM_CHAIN_STEP(1, 0)
-                "1:\n"
-                "s_mov_b64 exec, %[sv]\n"
-                : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1]), [l2] "+v"(key[2]),
-                  [l3] "+v"(key[3]), [l4] "+v"(key[4]), [l5] "+v"(key[5]), [l6] "+v"(key[6]),
-                  [l7] "+v"(key[7]), [l8] "+v"(key[8]), [l9] "+v"(key[9])
-                : [k] "v"(kk)
-                : "vcc");
-        }
-#undef RM_CHAIN_STEP
+        if constexpr (K == 10 && NODUP)
+            RM_CHAIN(9, RM_DUP_10, RM_CS_10, RM_OPS_10, , [ki] "v"(key_idx(kk)),
+                     [i0] "v"(key_idx(key[0])), [i1] "v"(key_idx(key[1])),
+                     [i2] "v"(key_idx(key[2])), [i3] "v"(key_idx(key[3])),
+                     [i4] "v"(key_idx(key[4])), [i5] "v"(key_idx(key[5])),
+                     [i6] "v"(key_idx(key[6])), [i7] "v"(key_idx(key[7])),
+                     [i8] "v"(key_idx(key[8])));
+        else if constexpr (K == 10)
+            RM_CHAIN(9, "", RM_CS_10, RM_OPS_10);
+        else if constexpr (K == 2)
+            RM_CHAIN(1, "", RM_CS(1, 0), RM_OPS_2);
+        else if constexpr (K == 4)
+            RM_CHAIN(3, "", RM_CS_4, RM_OPS_4);
+        else if constexpr (K == 8)
+            RM_CHAIN(7, "", RM_CS_8, RM_OPS_8);
+        else if constexpr (K == 16)
+            RM_CHAIN(15, "", RM_CS_16, RM_OPS_16);
+        else if constexpr (K == 32)
+            RM_CHAIN(31, "", RM_CS_32, RM_OPS_32);
     }
-    // The chain for K = 16 (radius searches of up to 16 neighbours, kNN with k = 11..16).
-    __device__ __forceinline__ void insert_chain16(uint64_t kk) {
-        static_assert(K == 16, "the chain is written for sixteen slots");
-        uint64_t saved;
-#define RM_CHAIN_STEP(hi, lo)                              \
-    "v_cmpx_lt_u64_e32 vcc, %[k], %[l" #lo "]\n"           \
-    "s_cbranch_execz 1f\n"                                 \
-    "v_mov_b64_e32 %[l" #hi "], %[l" #lo "]\n"             \
-    "v_mov_b64_e32 %[l" #lo "], %[k]\n"
-        asm volatile(
-            "s_mov_b64 %[sv], exec\n"
-            "v_cmpx_lt_u64_e32 vcc, %[k], %[l15]\n"
-            "s_cbranch_execz 1f\n"
-            "v_mov_b64_e32 %[l15], %[k]\n"
-            RM_CHAIN_STEP(15, 14) RM_CHAIN_STEP(14, 13) RM_CHAIN_STEP(13, 12)
-            RM_CHAIN_STEP(12, 11) RM_CHAIN_STEP(11, 10) RM_CHAIN_STEP(10, 9) RM_CHAIN_STEP(9, 8)
-            RM_CHAIN_STEP(8, 7) RM_CHAIN_STEP(7, 6) RM_CHAIN_STEP(6, 5) RM_CHAIN_STEP(5, 4)
-            RM_CHAIN_STEP(4, 3) RM_CHAIN_STEP(3, 2) RM_CHAIN_STEP(2, 1) RM_CHAIN_STEP(1, 0)
-            "1:\n"
-            "s_mov_b64 exec, %[sv]\n"
-            : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1]), [l2] "+v"(key[2]),
-              [l3] "+v"(key[3]), [l4] "+v"(key[4]), [l5] "+v"(key[5]), [l6] "+v"(key[6]),
-              [l7] "+v"(key[7]), [l8] "+v"(key[8]), [l9] "+v"(key[9]), [l10] "+v"(key[10]),
-              [l11] "+v"(key[11]), [l12] "+v"(key[12]), [l13] "+v"(key[13]),
-              [l14] "+v"(key[14]), [l15] "+v"(key[15])
-            : [k] "v"(kk)
-            : "vcc");
-#undef RM_CHAIN_STEP
-    }
-    // The chain for K = 2 (the radii pass): at most two exec-masked steps.
-    __device__ __forceinline__ void insert_chain2(uint64_t kk) {
-        static_assert(K == 2, "the chain is written for two slots");
-        uint64_t saved;
-        asm volatile(
-            "s_mov_b64 %[sv], exec\n"
-            "v_cmpx_lt_u64_e32 vcc, %[k], %[l1]\n"
-            "s_cbranch_execz 1f\n"
-            "v_mov_b64_e32 %[l1], %[k]\n"
-            "v_cmpx_lt_u64_e32 vcc, %[k], %[l0]\n"
-            "s_cbranch_execz 1f\n"
-            "v_mov_b64_e32 %[l1], %[l0]\n"
-            "v_mov_b64_e32 %[l0], %[k]\n"
-            "1:\n"
-            "s_mov_b64 exec, %[sv]\n"
-            : [sv] "=&s"(saved), [l0] "+v"(key[0]), [l1] "+v"(key[1])
-            : [k] "v"(kk)
-            : "vcc");
-    }
+#undef RM_DUP_10
+#undef RM_CHAIN
+#undef RM_OPS_32
+#undef RM_OPS_16
+#undef RM_OPS_10
+#undef RM_OPS_8
+#undef RM_OPS_4
+#undef RM_OPS_2
+#undef RM_OP
+#undef RM_CS_32
+#undef RM_CS_16
+#undef RM_CS_10
+#undef RM_CS_8
+#undef RM_CS_4
+#undef RM_CS
     template <bool NODUP>
     __device__ __forceinline__ void add(uint64_t kk) {
-        if constexpr (K == 10)
+        // the chain checks duplicates for K = 10 only (the warm-started march); other lists
+        // that need the check take the select network
+        if constexpr (kChain && (!NODUP || K == 10))
             insert_chain<NODUP>(kk);
-        else if constexpr (K == 2 && !NODUP)
-            insert_chain2(kk);
-        else if constexpr (K == 16 && !NODUP)
-            insert_chain16(kk);
         else if (NODUP)
             insert_nodup(kk);
         else
