@@ -462,11 +462,7 @@ __device__ void knn_packet_rounds(const DevTree& t, float qx, float qy, float qz
         for (int i = 0; i < nq; ++i) {
             const int e = i == 0 ? q0 : q1;
             const int ni = e >> 2, s = (e >> 1) & 1;
-            CFloat* f = node_rec(nf, ni);
-            CInt* fi = (CInt*)f;
-            CFloat* bx = f + 6 * s;
-            const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
-            if ((__builtin_amdgcn_ballot_w64(md <= tk.worst_d2()) & amask) == 0) continue;
+            CInt* fi = (CInt*)node_rec(nf, ni);
             packet_scan_leaf<K, NODUP, CULL>(pf, fi[12 + s], fi[14 + s], qx, qy, qz, tk, mask);
         }
     }
