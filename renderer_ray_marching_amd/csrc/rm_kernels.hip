@@ -256,9 +256,10 @@ __device__ __forceinline__ int xcd_block() {
 
 // ------------------------------------------------------------------------------------------
 // Stack entries (both traversals): an inner child is pushed as (node << 1), a leaf child as
-// (parent << 2 | slot << 1 | 1). A leaf's box is re-tested against the current k-th distance
-// when it is popped (the bound may have shrunk since the push, and a leaf scan is costly); an
-// inner child is expanded directly, since expanding it tests both of its children anyway.
+// (parent << 2 | slot << 1 | 1). The per-lane walk re-tests a leaf's box against the current
+// k-th distance when it is popped (the bound may have shrunk since the push); the packet walks
+// scan a popped leaf directly, since the union of 64 lanes' balls nearly always still needs it.
+// An inner child is expanded directly, since expanding it tests both of its children anyway.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ int child_entry(int ni, int s, int ref, int cnt) {
     return cnt > 0 ? ((ni << 2) | (s << 1) | 1) : (ref << 1);
@@ -412,11 +413,7 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
         const int e = __builtin_amdgcn_readfirstlane(wstk[sp]);
         if (e & 1) {
             const int ni = e >> 2, s = (e >> 1) & 1;
-            CFloat* f = node_rec(nf, ni);
-            CInt* fi = (CInt*)f;
-            CFloat* bx = f + 6 * s;
-            const float md = mindist(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], qx, qy, qz);
-            if ((__builtin_amdgcn_ballot_w64(md <= tk.worst_d2()) & amask) == 0) continue;
+            CInt* fi = (CInt*)node_rec(nf, ni);
             const int b = fi[12 + s];
             if (b >= skip_b && b < skip_e) continue;
             packet_scan_leaf<K, NODUP, false, true>(pf, b, fi[14 + s], qx, qy, qz, tk);
@@ -428,7 +425,7 @@ __device__ void knn_packet(const DevTree& t, float qx, float qy, float qz, bool 
 
 // Same result as knn_packet, in rounds: expand inner nodes (reading only the k-th distance)
 // until the stack is empty or kLeafQueue leaves are queued, then scan the queued leaves in
-// order, re-testing each leaf's box against the current k-th distance. The list is modified in
+// order. The list is modified in
 // the scan loop only, so it keeps its registers across the node expansions. Leaves are queued
 // in the near-first order of the depth-first walk; bounds tightened by a round prune the
 // entries still on the stack when they are expanded.
