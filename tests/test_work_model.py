@@ -79,6 +79,13 @@ def test_packet_walk_equals_the_oracle_and_its_counts_add_up():
         assert c.expansions >= c.wave_steps and c.containing_steps <= c.wave_steps
         # a seeded list only gains points it does not hold; one seeded with bounds holds none
         assert c.lane_inserts >= (10 * c.lane_steps if bound_only else 0)
+        # insertion chain: a wavefront insert writes 0..10 slots (at least one unless every
+        # lane's list stays as it was); sorting a seed by insertion moves its keys at most
+        # 45 places (ten keys in reverse order)
+        moving = c.wave_inserts - c.inserts_no_move
+        assert 0 <= moving and moving <= c.insert_slots <= 10 * moving
+        assert (c.insert_slots > 0) == (c.lane_inserts > 0)
+        assert c.seed_steps <= c.wave_steps and c.seed_moves <= 45 * c.seed_steps
     # every 3rd tile only (tiles 0, 3, ..., 78 of 10 x 8; the last tile row is 4 pixels tall):
     # exactly their pixels are marched, the others keep -2
     h, c = wt.render_packet(v.pose_colmajor(), pane, 80, 60, 3)
